@@ -1,0 +1,116 @@
+"""Seeded synthetic genomes and long reads for the daligner overlap workloads.
+
+Everything here is deterministic in its seed (numpy PCG64), so the same call
+produces the same reads on any machine with the same numpy.  Bases are coded
+0..3 = A,C,G,T as in the DB format (db/DB.c:261-281).
+
+Error model (per template base, independent):
+  * insertion: with probability p_ins a uniformly random base is emitted
+    before the template base;
+  * deletion:  with probability p_del the template base is dropped;
+  * substitution: with probability p_sub the template base is replaced by one
+    of the three other bases (uniform);
+  * otherwise the template base is copied.
+Deletion and substitution are exclusive outcomes of one uniform draw; the
+insertion draw is separate.  Reads are taken from a uniform start on the
+genome and reverse-complemented with probability 1/2.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class ReadSet:
+    reads: list            # list of np.uint8 arrays (0..3)
+    starts: np.ndarray     # template start on the genome
+    lens_t: np.ndarray     # template length
+    strand: np.ndarray     # 1 = reverse complement
+
+
+def random_genome(length: int, seed: int) -> np.ndarray:
+    """iid uniform ACGT genome of `length` bases."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.integers(0, 4, size=length, dtype=np.uint8)
+
+
+def lognormal_params(mean: float, sd: float):
+    s2 = math.log(1.0 + (sd * sd) / (mean * mean))
+    return math.log(mean) - s2 / 2.0, math.sqrt(s2)
+
+
+def mutate(template: np.ndarray, rng, p_ins: float, p_del: float, p_sub: float) -> np.ndarray:
+    n = template.shape[0]
+    u = rng.random(n)
+    ins = rng.random(n) < p_ins
+    ins_base = rng.integers(0, 4, size=n, dtype=np.uint8)
+    sub_off = rng.integers(1, 4, size=n, dtype=np.uint8)
+    keep = u >= p_del
+    sub = (u >= p_del) & (u < p_del + p_sub)
+    base = template.copy()
+    base[sub] = (base[sub] + sub_off[sub]) & 3
+    # interleave: [ins_base if ins] then [base if keep], per template position
+    cnt = ins.astype(np.int64) + keep.astype(np.int64)
+    out = np.empty(int(cnt.sum()), dtype=np.uint8)
+    pos = np.cumsum(cnt) - cnt
+    out[pos[ins]] = ins_base[ins]
+    out[(pos + ins.astype(np.int64))[keep]] = base[keep]
+    return out
+
+
+def simulate_reads(genome: np.ndarray, coverage: float, mean_len: float, sd_len: float,
+                   min_len: int, seed: int, p_ins=0.09, p_del=0.04, p_sub=0.02,
+                   max_len: int | None = None) -> ReadSet:
+    """Reads at `coverage`x with lognormal template lengths (mean, sd), clipped
+    below at `min_len` by resampling and above at `max_len` (and the genome)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mu, sigma = lognormal_params(mean_len, sd_len)
+    G = genome.shape[0]
+    target = coverage * G
+    total = 0
+    reads, starts, lens, strands = [], [], [], []
+    while total < target:
+        L = int(rng.lognormal(mu, sigma))
+        if L < min_len or L > G or (max_len is not None and L > max_len):
+            continue
+        s = int(rng.integers(0, G - L + 1))
+        t = genome[s:s + L]
+        r = mutate(t, rng, p_ins, p_del, p_sub)
+        st = int(rng.integers(0, 2))
+        if st:
+            r = (3 - r[::-1]).astype(np.uint8)
+        reads.append(r)
+        starts.append(s)
+        lens.append(L)
+        strands.append(st)
+        total += L
+    return ReadSet(reads, np.array(starts), np.array(lens), np.array(strands))
+
+
+def write_fasta(path: str, reads, prefix: str = "sim", width: int = 80) -> None:
+    """PacBio-style headers `>prefix/<well>/0_<len>` so FA2db keeps one read per well."""
+    letters = np.frombuffer(b"ACGT", dtype=np.uint8)
+    with open(path, "wb") as f:
+        for i, r in enumerate(reads):
+            f.write(b">%s/%d/0_%d\n" % (prefix.encode(), i + 1, len(r)))
+            s = letters[r].tobytes()
+            for j in range(0, len(s), width):
+                f.write(s[j:j + width] + b"\n")
+
+
+# ---- the named workloads -------------------------------------------------------
+
+def config_small(seed: int = 1):
+    """200 kbp iid genome, 20x, lognormal(4 kb, 2 kb, min 500), 9/4/2 % errors."""
+    g = random_genome(200_000, seed)
+    return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
+
+
+def config_5mbp(seed: int = 2):
+    """5 Mbp iid genome, 30x, lognormal(10 kb, 6 kb, min 1 kb), 9/4/2 % errors."""
+    g = random_genome(5_000_000, seed)
+    return simulate_reads(g, 30.0, 10000.0, 6000.0, 1000, seed + 1000)

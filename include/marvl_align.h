@@ -1,0 +1,147 @@
+/* libmarvl_align: MI355X (gfx950) implementation of MARVEL daligner's
+ * overlap hot path behind a plain C ABI.
+ *
+ * Two groups of entry points:
+ *   1. the names and signatures of MARVEL's dalign/align.h that the daligner
+ *      path and the downstream tools bind (Local_Alignment and friends), so a
+ *      C host links this library in place of marvl-align;
+ *   2. batched entry points with no reference counterpart (mi_*), used by the
+ *      GPU daligner driver, the tests and the benchmark.
+ * Every function returns 0 / non-NULL on success.  Errors of the mi_* calls
+ * are negative codes, with a message from mi_last_error().  The align.h
+ * drop-ins keep the reference's batch-mode behaviour: they print a message
+ * and exit(1) (db/DB.h:32-36).
+ */
+#ifndef MARVL_ALIGN_H
+#define MARVL_ALIGN_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference-compatible types (dalign/align.h:76-117, :280-336) ----- */
+
+typedef struct {
+    void *trace;
+    int tlen;
+    int diffs;
+    int abpos, bbpos;
+    int aepos, bepos;
+} Path;
+
+typedef struct {
+    Path *path;
+    uint32_t flags;
+    char *aseq;
+    char *bseq;
+    int alen;
+    int blen;
+} Alignment;
+
+typedef struct {
+    Path path;
+    uint32_t flags;
+    int aread;
+    int bread;
+} Overlap;
+
+typedef void Work_Data;
+typedef void Align_Spec;
+
+#define COMP(x) ((x)&0x1)
+#define COMP_FLAG 0x1
+#define TRACE_XOVR 125
+
+/* New_Align_Spec: replaces dalign/align.c:214-279 (decl. dalign/align.h:160).
+ * Same integer parameters (ave_path, match/mismatch scores, 15-bit trim tables). */
+Align_Spec *New_Align_Spec(double ave_corr, int trace_space, float *freq, int nthreads, int symmetric,
+                           int only_identity, int no_trace_points);
+/* dalign/align.h:162-166 */
+void Free_Align_Spec(Align_Spec *spec);
+int Trace_Spacing(Align_Spec *spec);
+double Average_Correlation(Align_Spec *spec);
+float *Base_Frequencies(Align_Spec *spec);
+
+/* New_Work_Data / Free_Work_Data: replace dalign/align.c:64-130 (dalign/align.h:131-133). */
+Work_Data *New_Work_Data(void);
+void Free_Work_Data(Work_Data *work);
+
+/* Local_Alignment: replaces dalign/align.c:1922-2060 (dalign/align.h:195).
+ * Runs on the GPU (one wave64).  Same contract: align->path receives the
+ * A-view path, the returned Path is the B-view path, both traces (uint16
+ * pairs) live in `work` until the next call.  aseq/bseq must point into
+ * buffers holding the value 4 right before and right after each read. */
+Path *Local_Alignment(Alignment *align, Work_Data *work, Align_Spec *spec, int low, int hgh, int anti, int lbord,
+                      int hbord);
+
+/* Overlap record helpers: replace dalign/align.c:3328-3360, :3419-3449
+ * (dalign/align.h:309-318); host code, byte-compatible .las records. */
+int Read_Overlap(FILE *input, Overlap *ovl);
+int Read_Trace(FILE *input, Overlap *ovl, int tbytes);
+void Write_Overlap(FILE *output, Overlap *ovl, int tbytes);
+void Compress_TraceTo8(Overlap *ovl);
+void Decompress_TraceTo16(Overlap *ovl);
+int Check_Trace_Points(Overlap *ovl, int tspace, int verbose, char *fname);
+
+/* ---- batched entry points (no reference counterpart) ------------------- */
+
+#define MI_ERR_ARG -1
+#define MI_ERR_DEVICE -2
+#define MI_ERR_UNDEF -3
+#define MI_ERR_TIMEOUT -4   /* a kernel outran the watchdog deadline (MI_WATCHDOG_S) and was stopped */
+
+/* Alignment parameters as New_Align_Spec takes them. */
+typedef struct {
+    double ave_corr;
+    int trace_space;
+    float freq[4];
+} MiAlignParams;
+
+/* One Local_Alignment task.  Reads are 2-bit packed in one uint64 buffer:
+ * base i of a read at bits 2*(i&31) of word (aword + (i>>5)); one zero guard
+ * word before aword and two after the read's last word. */
+typedef struct {
+    long long aword, bword;
+    int alen, blen;
+    int comp;             /* B read is the reverse complement (COMP flag) */
+    int low, hgh, anti;   /* Local_Alignment's low, hgh, anti */
+    int lbord, hbord;     /* Local_Alignment's lbord, hbord */
+    int selfie;           /* A and B are the same sequence object (aseq == bseq) */
+} MiLaTask;
+
+typedef struct {
+    int status;                             /* 0 ok, 3: reference-undefined wave state */
+    int abpos, bbpos, aepos, bepos, diffs;  /* A-view path */
+    int b_abpos, b_bbpos, b_aepos, b_bepos; /* B-view path (same diffs) */
+    int atlen, btlen;                       /* trace lengths (uint16 values) */
+    long long atrace, btrace;               /* offsets of the traces in the output trace array */
+    long long cells, snake;                 /* work counters: wave widths, snake compare steps */
+} MiLaResult;
+
+/* Runs every task on the GPU; traces are appended to `traces` (capacity
+ * ntrace uint16 values) in task order, A then B. */
+int mi_la_batch_host(const MiAlignParams *prm, const uint64_t *packed, long long npacked, const MiLaTask *tasks,
+                     int ntasks, MiLaResult *results, uint16_t *traces, long long ntrace);
+
+/* Alignment parameters New_Align_Spec derives (ave_path, mscore, dscore). */
+int mi_align_params(const MiAlignParams *prm, int *ave_path, int *mscore, int *dscore);
+
+/* Last error message of this thread. */
+const char *mi_last_error(void);
+
+/* Number of GPUs visible (0 when none); never initialises more than the runtime. */
+int mi_device_count(void);
+
+/* Device self-tests of the wave64 building blocks (0: persistent task loop,
+ * 1: lane scans / shuffles / ballots, 2: 2-bit snake matcher); the caller
+ * checks `out` against its own CPU computation. */
+int mi_selftest(int which, const void *in, long long nin, const void *in2, long long nin2, const void *q, int nq,
+                int arg, void *out, long long nout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

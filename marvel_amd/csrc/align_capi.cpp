@@ -1,0 +1,178 @@
+// dalign/align.h drop-in entry points of libmarvl_align (host side).
+// Local_Alignment forwards to the GPU batch path with one task; the record
+// helpers are plain host code on the byte-compatible .las layout.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/marvl_align.h"
+#include "mi_common.h"
+
+namespace {
+
+struct SpecObj {
+    mi::HostSpec hs;
+    MiAlignParams prm;
+    int nthreads, symmetric, only_identity, no_trace;
+};
+
+struct WorkObj {
+    std::vector<uint64_t> packed;
+    std::vector<uint16_t> traces;
+    Path bpath;
+};
+
+[[noreturn]] void die(const char* what)
+{
+    fprintf(stderr, "libmarvl_align: %s (%s)\n", what, mi_last_error());
+    exit(1);
+}
+
+// 2-bit pack `len` bases (values 0..3) into words starting at out[at]
+void pack_into(std::vector<uint64_t>& out, long long at, const char* s, int len)
+{
+    for (int i = 0; i < len; i++) out[at + (i >> 5)] |= (uint64_t)(s[i] & 3) << (2 * (i & 31));
+}
+
+}  // namespace
+
+extern "C" {
+
+Align_Spec* New_Align_Spec(double ave_corr, int trace_space, float* freq, int nthreads, int symmetric,
+                           int only_identity, int no_trace_points)
+{
+    SpecObj* s = new SpecObj;
+    mi::spec_build(&s->hs, ave_corr, trace_space, freq);
+    s->prm.ave_corr = ave_corr;
+    s->prm.trace_space = trace_space;
+    for (int i = 0; i < 4; i++) s->prm.freq[i] = freq[i];
+    s->nthreads = nthreads;
+    s->symmetric = symmetric;
+    s->only_identity = only_identity;
+    s->no_trace = no_trace_points;
+    return (Align_Spec*)s;
+}
+
+void Free_Align_Spec(Align_Spec* spec) { delete (SpecObj*)spec; }
+int Trace_Spacing(Align_Spec* spec) { return ((SpecObj*)spec)->hs.tspace; }
+double Average_Correlation(Align_Spec* spec) { return ((SpecObj*)spec)->hs.ave_corr; }
+float* Base_Frequencies(Align_Spec* spec) { return ((SpecObj*)spec)->hs.freq; }
+
+Work_Data* New_Work_Data(void) { return (Work_Data*)new WorkObj; }
+void Free_Work_Data(Work_Data* work) { delete (WorkObj*)work; }
+
+Path* Local_Alignment(Alignment* align, Work_Data* work, Align_Spec* spec, int low, int hgh, int anti, int lbord,
+                      int hbord)
+{
+    SpecObj* s = (SpecObj*)spec;
+    WorkObj* w = (WorkObj*)work;
+    const int alen = align->alen, blen = align->blen;
+    const long long aw = (alen + 31) / 32, bw = (blen + 31) / 32;
+    const bool selfie = (align->aseq == align->bseq);
+    // layout: [guard][A words][guard guard][B words][guard guard]
+    const long long a0 = 1, b0 = selfie ? 1 : a0 + aw + 2;
+    w->packed.assign((selfie ? a0 + aw : b0 + bw) + 2, 0);
+    pack_into(w->packed, a0, align->aseq, alen);
+    if (!selfie) pack_into(w->packed, b0, align->bseq, blen);
+    MiLaTask t;
+    t.aword = a0;
+    t.bword = b0;
+    t.alen = alen;
+    t.blen = blen;
+    t.comp = COMP(align->flags) ? 1 : 0;
+    t.low = low;
+    t.hgh = hgh;
+    t.anti = anti;
+    t.lbord = lbord;
+    t.hbord = hbord;
+    t.selfie = selfie ? 1 : 0;
+    const int mx = alen > blen ? alen : blen;
+    const long long cap = 4LL * (mx / s->hs.tspace + 2) + 16;
+    w->traces.assign(cap, 0);
+    MiLaResult r;
+    int rc = mi_la_batch_host(&s->prm, w->packed.data(), (long long)w->packed.size(), &t, 1, &r, w->traces.data(), cap);
+    if (rc != 0) die("Local_Alignment failed");
+    if (r.status != 0) {
+        mi::mi_set_error("wave reached a state the reference leaves undefined");
+        die("Local_Alignment");
+    }
+    Path* ap = align->path;
+    ap->abpos = r.abpos;
+    ap->bbpos = r.bbpos;
+    ap->aepos = r.aepos;
+    ap->bepos = r.bepos;
+    ap->diffs = r.diffs;
+    ap->tlen = r.atlen;
+    ap->trace = w->traces.data() + r.atrace;
+    Path* bp = &w->bpath;
+    bp->abpos = r.b_abpos;
+    bp->bbpos = r.b_bbpos;
+    bp->aepos = r.b_aepos;
+    bp->bepos = r.b_bepos;
+    bp->diffs = r.diffs;
+    bp->tlen = r.btlen;
+    bp->trace = w->traces.data() + r.btrace;
+    return bp;
+}
+
+// ---- .las record helpers (dalign/align.c:3325-3449) -------------------------
+// A record on disk is the Overlap struct without its leading trace pointer:
+// 40 bytes = tlen, diffs, abpos, bbpos, aepos, bepos, flags, aread, bread, pad.
+
+static const size_t kRecBytes = sizeof(Overlap) - sizeof(void*);
+
+int Read_Overlap(FILE* input, Overlap* ovl)
+{
+    return fread(((char*)ovl) + sizeof(void*), kRecBytes, 1, input) != 1;
+}
+
+int Read_Trace(FILE* input, Overlap* ovl, int tbytes)
+{
+    if (tbytes > 0 && ovl->path.tlen > 0)
+        if (fread(ovl->path.trace, (size_t)tbytes * ovl->path.tlen, 1, input) != 1) return 1;
+    return 0;
+}
+
+void Write_Overlap(FILE* output, Overlap* ovl, int tbytes)
+{
+    fwrite(((char*)ovl) + sizeof(void*), kRecBytes, 1, output);
+    if (ovl->path.trace != NULL) fwrite(ovl->path.trace, tbytes, ovl->path.tlen, output);
+}
+
+void Compress_TraceTo8(Overlap* ovl)
+{
+    uint16_t* t16 = (uint16_t*)ovl->path.trace;
+    uint8_t* t8 = (uint8_t*)ovl->path.trace;
+    for (int j = 0; j < ovl->path.tlen; j++) t8[j] = (uint8_t)t16[j];
+}
+
+void Decompress_TraceTo16(Overlap* ovl)
+{
+    uint16_t* t16 = (uint16_t*)ovl->path.trace;
+    uint8_t* t8 = (uint8_t*)ovl->path.trace;
+    for (int j = ovl->path.tlen - 1; j >= 0; j--) t16[j] = t8[j];
+}
+
+int Check_Trace_Points(Overlap* ovl, int tspace, int verbose, char* fname)
+{
+    if (((ovl->path.aepos - 1) / tspace - ovl->path.abpos / tspace) * 2 != ovl->path.tlen - 2) {
+        if (verbose) fprintf(stderr, "  %s: Wrong number of trace points\n", fname);
+        return 1;
+    }
+    int p = ovl->path.bbpos;
+    if (tspace <= TRACE_XOVR) {
+        uint8_t* t = (uint8_t*)ovl->path.trace;
+        for (int i = 1; i < ovl->path.tlen; i += 2) p += t[i];
+    } else {
+        uint16_t* t = (uint16_t*)ovl->path.trace;
+        for (int i = 1; i < ovl->path.tlen; i += 2) p += t[i];
+    }
+    if (p != ovl->path.bepos) {
+        if (verbose) fprintf(stderr, "  %s: Trace point sum != aligned interval\n", fname);
+        return 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// Internal helpers shared by the HIP translation units of libmarvl_align.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/marvl_align.h"
+#include "la_wave.hpp"
+
+namespace mi {
+
+// Host-derived alignment constants (dalign/align.c:214-279) and their device copy.
+struct HostSpec {
+    double ave_corr;
+    int tspace;
+    float freq[4];
+    int ave_path, mscore, dscore;
+    int16_t table[32768];
+    int16_t score[32768];
+};
+
+struct DevSpecHost {
+    HostSpec host;
+    DevSpec spec;   // table/score point at device memory
+};
+
+void spec_build(HostSpec* s, double ave_corr, int tspace, const float* freq);
+// Device tables for these parameters (cached per parameter set; lives until exit).
+DevSpecHost* mi_spec_device(const MiAlignParams* prm);
+int mi_num_cus();
+void mi_set_error(const char* msg);
+// Watchdog: device pointer of a host-pinned flag every long-running kernel
+// polls; mi_wait_watchdog waits for stream `st` and, past the deadline
+// (MI_WATCHDOG_S seconds, default 120), raises the flag so the kernel's waves
+// exit, then reports the timeout.  The flag is cleared before returning.
+const int* mi_abort_flag_device();
+int mi_wait_watchdog(hipStream_t st, const char* what);
+
+}  // namespace mi

@@ -1,0 +1,83 @@
+"""ctypes binding of libmarvl_align.so (declarations: include/marvl_align.h)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmarvl_align.so")
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class MiAlignParams(C.Structure):
+    _fields_ = [("ave_corr", C.c_double), ("trace_space", C.c_int), ("freq", C.c_float * 4)]
+
+
+class MiLaTask(C.Structure):
+    _fields_ = [("aword", C.c_longlong), ("bword", C.c_longlong), ("alen", C.c_int), ("blen", C.c_int),
+                ("comp", C.c_int), ("low", C.c_int), ("hgh", C.c_int), ("anti", C.c_int),
+                ("lbord", C.c_int), ("hbord", C.c_int), ("selfie", C.c_int)]
+
+
+class MiLaResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("abpos", C.c_int), ("bbpos", C.c_int), ("aepos", C.c_int),
+                ("bepos", C.c_int), ("diffs", C.c_int), ("b_abpos", C.c_int), ("b_bbpos", C.c_int),
+                ("b_aepos", C.c_int), ("b_bepos", C.c_int), ("atlen", C.c_int), ("btlen", C.c_int),
+                ("atrace", C.c_longlong), ("btrace", C.c_longlong), ("cells", C.c_longlong),
+                ("snake", C.c_longlong)]
+
+
+# every symbol declared in include/marvl_align.h (checked by the CPU tests)
+EXPORTS = [
+    "New_Align_Spec", "Free_Align_Spec", "Trace_Spacing", "Average_Correlation", "Base_Frequencies",
+    "New_Work_Data", "Free_Work_Data", "Local_Alignment",
+    "Read_Overlap", "Read_Trace", "Write_Overlap", "Compress_TraceTo8", "Decompress_TraceTo16",
+    "Check_Trace_Points",
+    "mi_la_batch_host", "mi_align_params", "mi_last_error", "mi_device_count", "mi_selftest",
+]
+
+
+class _Lib:
+    def __init__(self):
+        self._h = None
+
+    def handle(self) -> C.CDLL:
+        if self._h is None:
+            if not os.path.exists(LIB_PATH):
+                raise LibraryMissing(f"{LIB_PATH} not built: run `python -m marvel_amd.build`")
+            h = C.CDLL(LIB_PATH)
+            h.mi_la_batch_host.argtypes = [C.POINTER(MiAlignParams), C.c_void_p, C.c_longlong,
+                                           C.POINTER(MiLaTask), C.c_int, C.POINTER(MiLaResult),
+                                           C.c_void_p, C.c_longlong]
+            h.mi_la_batch_host.restype = C.c_int
+            h.mi_align_params.argtypes = [C.POINTER(MiAlignParams), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]
+            h.mi_last_error.restype = C.c_char_p
+            h.mi_device_count.restype = C.c_int
+            h.mi_selftest.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p,
+                                      C.c_int, C.c_int, C.c_void_p, C.c_longlong]
+            h.mi_selftest.restype = C.c_int
+            self._h = h
+        return self._h
+
+    def __getattr__(self, name):
+        return getattr(self.handle(), name)
+
+    def last_error(self) -> str:
+        return self.handle().mi_last_error().decode()
+
+
+lib = _Lib()
+
+
+def params(ave_corr: float, trace_space: int, freq) -> MiAlignParams:
+    p = MiAlignParams()
+    p.ave_corr = ave_corr
+    p.trace_space = trace_space
+    for i in range(4):
+        p.freq[i] = float(freq[i])
+    return p

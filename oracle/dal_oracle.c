@@ -92,6 +92,7 @@ struct or_work {
     uint16_t *tr;        /* trace workspace: [rev A | fwd A][rev B | fwd B] */
     int trcap;
     int64_t cells, snake;
+    int64_t nwaves, maxw, hist[9];   /* wave-width statistics: widths 1-64,65-128,...,>512 */
 };
 
 or_work *or_work_new(void)
@@ -113,6 +114,21 @@ void or_work_counters(const or_work *w, int64_t *cells, int64_t *snake)
 {
     *cells = w->cells;
     *snake = w->snake;
+}
+
+static inline void wstat(or_work *w, int width)
+{
+    int b = (width - 1) >> 6;
+    w->nwaves += 1;
+    if (width > w->maxw) w->maxw = width;
+    w->hist[b > 8 ? 8 : b] += 1;
+}
+
+void or_work_widths(const or_work *w, int64_t *nwaves, int64_t *maxw, int64_t hist[9])
+{
+    *nwaves = w->nwaves;
+    *maxw = w->maxw;
+    for (int i = 0; i < 9; i++) hist[i] = w->hist[i];
 }
 
 #define BAND_GUARD 256
@@ -180,6 +196,7 @@ typedef struct {
 static inline int snake(const char *A, const char *B, int k, int y, int dir, int *stop, int *steps)
 {
     int y0 = y;
+    (void)dir;
     if (dir > 0) {
         for (;;) {
             int c = B[y];
@@ -256,6 +273,7 @@ static int fwd_pass(const or_spec *sp, or_work *w, const char *A, int alen, cons
         int hb = peb_push(w, -1, k, 0, nb);
         nb += TS;
 
+        if (y < 0 || y > blen || y + k < 0 || y + k > alen) return BAD_INPUT;
         y = snake(A, B, k, y, +1, &stop, &st);
         w->snake += st + 1;
         if (stop == 1) { s.more = 0; if (s.bclip < k) s.bclip = k; }
@@ -335,6 +353,7 @@ static int fwd_pass(const or_spec *sp, or_work *w, const char *A, int alen, cons
         n = W_PATH_LEN;
         ua = ub = -1;
         w->cells += s.hgh - s.low + 1;
+        wstat(w, s.hgh - s.low + 1);
 
         for (k = s.hgh; k >= s.low; k--) {
             int c, m, ha, hb, y, stop, st;
@@ -350,6 +369,7 @@ static int fwd_pass(const or_spec *sp, or_work *w, const char *A, int alen, cons
             if ((b & W_PATH_TOP) != 0) m -= 1;
             b <<= 1;
             y = (c - k) >> 1;
+            if (y < 0 || y > blen || y + k < 0 || y + k > alen) return BAD_INPUT;
             y = snake(A, B, k, y, +1, &stop, &st);
             w->snake += st + 1;
             apply_matches(&m, &b, st);
@@ -493,6 +513,7 @@ static int rev_pass(const or_spec *sp, or_work *w, const char *A0, int alen, con
         int nb = ((y + (TS - boff) - 1) / TS - 1) * TS + boff;
         int hb = peb_push(w, -1, k, 0, y);
 
+        if (y < 0 || y > blen || y + k < 0 || y + k > alen) return BAD_INPUT;
         /* forward-style snake on the shifted views: compare B[y] / A[y+k] and step down */
         int y0 = y;
         for (;;) {
@@ -578,6 +599,7 @@ static int rev_pass(const or_spec *sp, or_work *w, const char *A0, int alen, con
         n = W_PATH_LEN;
         ua = ub = -1;
         w->cells += s.hgh - s.low + 1;
+        wstat(w, s.hgh - s.low + 1);
 
         for (k = s.low; k <= s.hgh; k++) {
             int c, m, ha, hb, y, stop, st;
@@ -593,6 +615,7 @@ static int rev_pass(const or_spec *sp, or_work *w, const char *A0, int alen, con
             if ((b & W_PATH_TOP) != 0) m -= 1;
             b <<= 1;
             y = (c - k) >> 1;
+            if (y < 0 || y > blen || y + k < 0 || y + k > alen) return BAD_INPUT;
             int y0 = y;
             for (;;) {
                 int cc = B[y];
@@ -1638,6 +1661,58 @@ int or_daligner_pair(const or_opts *o, const or_spec *sp, or_block *A, or_block 
         or_block_free(&cB);
     }
     free(as);
+    if (o->verbose > 1) {
+        fprintf(stderr, "waves %lld maxw %lld hist", (long long)w->nwaves, (long long)w->maxw);
+        for (int i = 0; i < 9; i++) fprintf(stderr, " %lld", (long long)w->hist[i]);
+        fprintf(stderr, "\n");
+    }
     or_work_free(w);
     return 0;
+}
+
+/* ======================================================================== */
+/* Flat task-array driver for tests: same task / result layout as
+ * oracle/la_vectors.c.  Returns the number of tasks whose wave reached a
+ * state the reference leaves undefined (their result status is 3). */
+int or_la_tasks(double ave_corr, int tspace, const float freq[4], const char *seq, int ntasks,
+                const int64_t *aoff, const int64_t *boff, const int32_t *par /* ntasks x 8 */,
+                int32_t *res /* ntasks x 13: 12 fields + status */, uint16_t *traces, int64_t *toff /* ntasks+1 */,
+                int64_t tcap)
+{
+    or_spec *sp = (or_spec *)malloc(sizeof(or_spec));
+    or_work *w = or_work_new();
+    int bad = 0;
+    int64_t t = 0;
+    or_spec_init(sp, ave_corr, tspace, freq);
+    for (int i = 0; i < ntasks; i++) {
+        const int32_t *v = par + 8 * i;
+        or_path pa, pb;
+        const uint16_t *ta = NULL, *tb = NULL;
+        int rc = or_local_alignment(sp, w, seq + aoff[i], v[0], seq + boff[i], v[1], v[2], v[3], v[4], v[5], v[6],
+                                    v[7], &pa, &ta, &pb, &tb);
+        int32_t *r = res + 13 * i;
+        toff[i] = t;
+        if (rc) {
+            memset(r, 0, sizeof(int32_t) * 13);
+            r[12] = rc;
+            bad += 1;
+            continue;
+        }
+        int32_t vals[13] = {pa.abpos, pa.bbpos, pa.aepos, pa.bepos, pa.diffs, pa.tlen,
+                            pb.abpos, pb.bbpos, pb.aepos, pb.bepos, pb.diffs, pb.tlen, 0};
+        memcpy(r, vals, sizeof vals);
+        if (t + pa.tlen + pb.tlen > tcap) {
+            free(sp);
+            or_work_free(w);
+            return -1;
+        }
+        memcpy(traces + t, ta, 2 * pa.tlen);
+        t += pa.tlen;
+        memcpy(traces + t, tb, 2 * pb.tlen);
+        t += pb.tlen;
+    }
+    toff[ntasks] = t;
+    free(sp);
+    or_work_free(w);
+    return bad;
 }

@@ -55,6 +55,8 @@ int or_local_alignment(const or_spec *sp, or_work *w, const char *aseq, int alen
 /* Work counters of the last or_local_alignment: wave widths summed and snake
  * compare steps, forward + reverse (the W_align units). */
 void or_work_counters(const or_work *w, int64_t *cells, int64_t *snake);
+/* Wave-width statistics accumulated over the life of w. */
+void or_work_widths(const or_work *w, int64_t *nwaves, int64_t *maxw, int64_t hist[9]);
 
 /* ---- DB block (db/DB.c:360-585, :1349-1410) ---------------------------- */
 typedef struct {

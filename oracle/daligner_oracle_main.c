@@ -58,7 +58,7 @@ int main(int argc, char **argv)
     o.mem_limit = (uint64_t)sysconf(_SC_PHYS_PAGES) * (uint64_t)sysconf(_SC_PAGESIZE);
     while ((c = getopt(argc, argv, "vbOTAIk:w:h:t:M:e:l:s:H:D:m:r:j:")) != -1) {
         switch (c) {
-        case 'v': o.verbose = 1; break;
+        case 'v': o.verbose += 1; break;
         case 'T': o.no_trace = 1; break;
         case 'A': o.symmetric = 0; break;
         case 'k': o.kmer = atoi(optarg); break;

@@ -1,0 +1,89 @@
+// CPU emulation of the wave64 intrinsics used by marvel_amd/csrc/la_wave.hpp.
+// TEST INFRASTRUCTURE ONLY: runs the device code unchanged on 64 host
+// threads (one per lane) so its control flow and indexing can be checked and
+// debugged on the CPU.  Every cross-lane operation is a two-phase barrier
+// exchange, so all 64 lanes must reach the same sequence of them (true for
+// the aligner: its cross-lane operations sit in wave-uniform control flow).
+#pragma once
+#include <barrier>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+
+#define __device__
+#define __global__
+#define __forceinline__ inline
+#define __host__
+
+namespace emu {
+struct Ctx {
+    std::barrier<> bar{64};
+    long long slot[64];
+};
+extern Ctx* g;
+extern thread_local int lane;
+template <class T>
+inline T xchg(T v, int src)
+{
+    long long x = 0;
+    memcpy(&x, &v, sizeof(T));
+    g->slot[lane] = x;
+    g->bar.arrive_and_wait();
+    long long r = g->slot[src & 63];
+    g->bar.arrive_and_wait();
+    T out;
+    memcpy(&out, &r, sizeof(T));
+    return out;
+}
+}  // namespace emu
+
+inline int __lane_id() { return emu::lane; }
+template <class T>
+inline T __shfl(T v, int src, int = 64) { return emu::xchg(v, src); }
+template <class T>
+inline T __shfl_up(T v, unsigned d, int = 64)
+{
+    int s = emu::lane - (int)d;
+    T r = emu::xchg(v, s < 0 ? emu::lane : s);
+    return r;
+}
+template <class T>
+inline T __shfl_down(T v, unsigned d, int = 64)
+{
+    int s = emu::lane + (int)d;
+    return emu::xchg(v, s > 63 ? emu::lane : s);
+}
+template <class T>
+inline T __shfl_xor(T v, int m, int = 64) { return emu::xchg(v, emu::lane ^ m); }
+inline unsigned long long __ballot(int p)
+{
+    emu::g->slot[emu::lane] = p ? 1 : 0;
+    emu::g->bar.arrive_and_wait();
+    unsigned long long m = 0;
+    for (int i = 0; i < 64; i++)
+        if (emu::g->slot[i]) m |= 1ull << i;
+    emu::g->bar.arrive_and_wait();
+    return m;
+}
+inline int __any(int p) { return __ballot(p) != 0; }
+inline int __builtin_amdgcn_readlane(int v, int l) { return emu::xchg(v, l); }
+inline int __builtin_amdgcn_readfirstlane(int v) { return emu::xchg(v, 0); }
+inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned acc)
+{
+    int l = emu::lane < 32 ? emu::lane : 32;
+    unsigned long long below = (l >= 32) ? 0xffffffffull : ((1ull << l) - 1);
+    return acc + __builtin_popcountll(m & below);
+}
+inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned m, unsigned acc)
+{
+    int l = emu::lane < 32 ? 0 : emu::lane - 32;
+    unsigned long long below = (1ull << l) - 1;
+    return acc + __builtin_popcountll(m & below);
+}
+#define __builtin_amdgcn_fence(a, b) __atomic_thread_fence(__ATOMIC_SEQ_CST)
+inline void __builtin_amdgcn_wave_barrier() { emu::g->bar.arrive_and_wait(); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __ffsll(long long x) { return __builtin_ffsll(x); }
+inline int __clzll(long long x) { return x ? __builtin_clzll((unsigned long long)x) : 64; }
+using std::max;
+using std::min;

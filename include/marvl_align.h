@@ -86,6 +86,33 @@ void Compress_TraceTo8(Overlap *ovl);
 void Decompress_TraceTo16(Overlap *ovl);
 int Check_Trace_Points(Overlap *ovl, int tspace, int verbose, char *fname);
 
+/* Per-thread overlap buffers of an Align_Spec (dalign/align.h:324-336). */
+typedef struct {
+    int tbytes;
+    uint64_t tmax;
+    uint64_t ttop;
+    void *trace;
+    int no_trace;
+    int omax;
+    int otop;
+    Overlap *ovls;
+} Overlap_IO_Buffer;
+
+/* Replace dalign/align.c:5630-5813 (dalign/align.h:338-351). */
+Overlap_IO_Buffer *CreateOverlapBuffer(int nthreads, int tbytes, int no_trace);
+int AddOverlapToBuffer(Overlap_IO_Buffer *iobuf, Overlap *ovl, int tbytes);
+Overlap_IO_Buffer *OVL_IO_Buffer(Align_Spec *espec);
+int Num_Threads(Align_Spec *espec);
+int Only_Identity(Align_Spec *espec);
+int Symmetric(Align_Spec *espec);
+/* Write_Overlap_Buffer: replaces dalign/align.c:5815-6043.  Stable sort of
+ * all buffered records by (aread, bread, COMP, abpos), then one .las file
+ * (self or not symmetric) or two split at lastRead, named
+ * [RRR_BBBBB/]A.a.B.b.las as the reference names them. */
+void Write_Overlap_Buffer(Align_Spec *spec, int RUN_ID, char *ablock, char *bblock, int lastRead);
+/* Reset_Overlap_Buffer: replaces dalign/align.c:6045-6056. */
+void Reset_Overlap_Buffer(Align_Spec *spec);
+
 /* ---- batched entry points (no reference counterpart) ------------------- */
 
 #define MI_ERR_ARG -1

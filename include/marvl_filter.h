@@ -1,0 +1,129 @@
+/* libmarvl_align: the k-mer match filter of MARVEL daligner on the GPU.
+ *
+ * Drop-in for dalign/filter.h: a C host that links this library in place of
+ * filter.o / align.o keeps calling Set_Filter_Params, Sort_Kmers and
+ * Match_Filter with the reference's arguments, and the records reach the
+ * Align_Spec's overlap buffers exactly as report_thread would have put them
+ * there (Write_Overlap_Buffer then writes byte-identical .las files).
+ *
+ * The block types below have the byte layout of db/DB.h:164-234 (HITS_READ
+ * 32 bytes, HITS_DB 88 bytes on x86-64), so blocks opened by the
+ * reference's Open_DB_Block / Read_All_Sequences can be passed unchanged.
+ */
+#ifndef MARVL_FILTER_H
+#define MARVL_FILTER_H
+
+#include <stdint.h>
+
+#include "marvl_align.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* db/DB.h:164-171 */
+typedef struct {
+    int rlen;       /* length of the read */
+    int64_t boff;   /* offset of the read's first base in HITS_DB.bases */
+    int64_t coff;
+    int flags;
+} HITS_READ;
+
+/* db/DB.h:213-234 (tracks are not used by the filter) */
+typedef struct {
+    int ureads;
+    float freq[4];
+    int maxlen;
+    int64_t totlen;
+    int nreads;
+    int part;
+    int ufirst;
+    char *path;
+    int loaded;
+    void *bases;        /* bytes 0..3 per base, with a 4 before and after every read */
+    HITS_READ *reads;   /* [0..nreads) */
+    void *tracks;
+} HITS_DB;
+
+/* Driver-owned parameters read by the filter (dalign/filter.h:54-73,
+ * defined at dalign/daligner.c:60-71).  The library defines them; a host
+ * that defines them itself takes precedence at link time. */
+extern int BIASED;
+extern int VERBOSE;
+extern int MINOVER;
+extern int HGAP_MIN;
+extern int SYMMETRIC;
+extern int IDENTITY;
+extern uint64_t MEM_LIMIT;
+extern uint64_t MEM_PHYSICAL;
+extern int NTHREADS;   /* report chunks (-j): fixes which pairs are skipped, so it is part of the output */
+extern int NSHIFT;     /* log2(NTHREADS) */
+
+/* Set_Filter_Params: replaces dalign/filter.c:119-139 (dalign/filter.h:67).
+ * Returns 1 for kmer <= 1.  This implementation also rejects kmer > 14. */
+int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin);
+
+/* Sort_Kmers: replaces dalign/filter.c:698-903 (dalign/filter.h:69).
+ * Builds the sorted k-mer index of `block` on the GPU and returns an opaque
+ * handle (the block's 2-bit packed reads stay on the device with it);
+ * *len = number of k-mers after -t suppression.  BIASED (-b) is not
+ * supported.  The handle is released by Match_Filter when passed as its
+ * btable with atable != btable (the reference frees its B index there,
+ * dalign/filter.c:2194-2202), otherwise by mi_free_kmer_index. */
+void *Sort_Kmers(HITS_DB *block, int *len);
+
+/* Match_Filter: replaces dalign/filter.c:2056-2407 (dalign/filter.h:71-73).
+ * Self comparison is decided like the reference: aname == bname as
+ * pointers.  The records of both views are appended to the overlap buffer
+ * of `asettings`.  Errors print a message and exit(1) like the reference's
+ * batch mode; mi_match_filter is the returning variant. */
+void Match_Filter(char *aname, HITS_DB *ablock, char *bname, HITS_DB *bblock, void *atable, int alen, void *btable,
+                  int blen, int comp, Align_Spec *asettings);
+
+/* ---- no reference counterpart ------------------------------------------- */
+
+/* Statistics of the last Match_Filter / mi_match_filter call. */
+typedef struct {
+    long long nkmers_a, nkmers_b;   /* index sizes */
+    long long nhits;                /* seeds after the -M cap */
+    long long npairs;               /* read pairs with seeds */
+    long long njobs;                /* pairs report_thread would scan */
+    long long naligned;             /* Local_Alignment calls */
+    long long nrecords;             /* records appended (both views) */
+    long long cells, snake;         /* aligner work: wave widths, snake compare steps */
+    long long edge_events;          /* band array accesses beyond [mindiag, maxdiag] */
+    long long edge_fixups;          /* pairs re-run for the band cell shared across pairs */
+    long long retries;              /* pairs re-run with wider windows / larger arenas */
+    int limit;                      /* k-mer match cap from MEM_LIMIT */
+    double t_filter, t_report, t_copy;   /* seconds: index merge + seed sort, report kernel, copy-out */
+    double t_total;
+} MiFilterStats;
+
+/* Match_Filter returning an error code (MI_ERR_*) instead of exiting. */
+int mi_match_filter(char *aname, HITS_DB *ablock, char *bname, HITS_DB *bblock, void *atable, void *btable,
+                    int comp, Align_Spec *asettings);
+void mi_filter_stats(MiFilterStats *out);
+
+/* Index of the reverse complement of the block indexed by `table`, built on
+ * the GPU from the packed reads the handle holds (the complement the
+ * driver makes with complement_DB, dalign/daligner.c:456-573, followed by
+ * Sort_Kmers).  `cblock` describes the complemented block (read lengths,
+ * ufirst, maxlen); its bases are not read. */
+void *mi_sort_kmers_complement(void *table, HITS_DB *cblock, int *len);
+void mi_free_kmer_index(void *table);
+
+/* Block loading for hosts without the reference's DB library: the
+ * behaviour of Open_DB_Block + Read_All_Sequences (db/DB.c:360-585,
+ * :1349-1410) for "DIR/ROOT.<block>" or "ROOT[.db]"; trimmed DBs and
+ * .dam files are not supported.  Returns 0 or MI_ERR_ARG. */
+int mi_open_block(const char *path, HITS_DB *db);
+void mi_close_block(HITS_DB *db);
+/* Header of the block's reverse complement for mi_sort_kmers_complement
+ * (read records shared with src, no bases, freq swapped as complement_DB
+ * swaps it, dalign/daligner.c:456-573). */
+void mi_complement_header(const HITS_DB *src, HITS_DB *dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -16,7 +16,9 @@ ARCH = os.environ.get("MI_OFFLOAD_ARCH", "gfx950")
 DEV_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
-LIB_SOURCES = ["la_batch.hip", "selftest.hip", "spec.cpp", "align_capi.cpp"]
+LIB_SOURCES = ["la_batch.hip", "filter.hip", "selftest.hip", "spec.cpp", "align_capi.cpp", "ovl_buffer.cpp",
+               "db_block.cpp"]
+CLI_SOURCE = "daligner_main.cpp"
 
 
 def _run(cmd):
@@ -46,6 +48,14 @@ def build(verbose: bool = False) -> str:
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
+    # the GPU daligner driver, linked against the library next to it
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exe = os.path.join(bindir, "daligner")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+          os.path.join(CSRC, CLI_SOURCE), "-o", exe + ".tmp", "-L", HERE, "-lmarvl_align",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -11,11 +11,7 @@
 
 namespace {
 
-struct SpecObj {
-    mi::HostSpec hs;
-    MiAlignParams prm;
-    int nthreads, symmetric, only_identity, no_trace;
-};
+using mi::SpecObj;
 
 struct WorkObj {
     std::vector<uint64_t> packed;
@@ -47,14 +43,31 @@ Align_Spec* New_Align_Spec(double ave_corr, int trace_space, float* freq, int nt
     s->prm.ave_corr = ave_corr;
     s->prm.trace_space = trace_space;
     for (int i = 0; i < 4; i++) s->prm.freq[i] = freq[i];
-    s->nthreads = nthreads;
+    s->nthreads = nthreads > 0 ? nthreads : 1;
     s->symmetric = symmetric;
     s->only_identity = only_identity;
     s->no_trace = no_trace_points;
+    // per-thread overlap buffers (dalign/align.c:260-276)
+    s->iobuf = (Overlap_IO_Buffer*)malloc(sizeof(Overlap_IO_Buffer) * s->nthreads);
+    for (int i = 0; i < s->nthreads; i++) {
+        Overlap_IO_Buffer* ob = CreateOverlapBuffer(s->nthreads, trace_space <= TRACE_XOVR ? 1 : 2, no_trace_points);
+        if (ob == nullptr) exit(1);
+        s->iobuf[i] = *ob;
+        free(ob);
+    }
     return (Align_Spec*)s;
 }
 
-void Free_Align_Spec(Align_Spec* spec) { delete (SpecObj*)spec; }
+void Free_Align_Spec(Align_Spec* spec)
+{
+    SpecObj* s = (SpecObj*)spec;
+    for (int i = 0; i < s->nthreads; i++) {
+        free(s->iobuf[i].ovls);
+        free(s->iobuf[i].trace);
+    }
+    free(s->iobuf);
+    delete s;
+}
 int Trace_Spacing(Align_Spec* spec) { return ((SpecObj*)spec)->hs.tspace; }
 double Average_Correlation(Align_Spec* spec) { return ((SpecObj*)spec)->hs.ave_corr; }
 float* Base_Frequencies(Align_Spec* spec) { return ((SpecObj*)spec)->hs.freq; }

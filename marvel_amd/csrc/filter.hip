@@ -1,0 +1,917 @@
+// MARVEL daligner's k-mer match filter (dalign/filter.c) on gfx950, behind
+// the reference's Set_Filter_Params / Sort_Kmers / Match_Filter interface.
+//
+// Sort_Kmers   : reads -> 2-bit packed words in HBM -> one 64-bit key per
+//                k-mer -> stable LSD radix sort on the code bits (keys are
+//                generated in (read, rpos) order, so sorting the code alone
+//                yields the reference's (code, read, rpos) order)
+// Match_Filter : merge-join of the A and B indexes by code (binary-searched
+//                group ranges, per-entry counts, prefix sums), the MEM_LIMIT
+//                cap from the count histogram, seed emission in merge order,
+//                stable radix sort of the seeds by (bread, aread, apos),
+//                split into read pairs, the report_thread chunk rule, then
+//                one wave per read pair (report.hpp): band scores, gating,
+//                Local_Alignment, redundancy fusion, records.
+// Records come back to the host and are appended to the Align_Spec's
+// overlap buffer in the order report_thread would have produced for equal
+// sort keys.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/marvl_align.h"
+#include "../../include/marvl_filter.h"
+#include "filter_kernels.hpp"
+#include "mi_common.h"
+#include "radix.hpp"
+#include "report.hpp"
+
+extern "C" {
+int BIASED = 0;
+int VERBOSE = 0;
+int MINOVER = 2000;
+int HGAP_MIN = 0;
+int SYMMETRIC = 1;
+int IDENTITY = 0;
+uint64_t MEM_LIMIT = 0;
+uint64_t MEM_PHYSICAL = 0;
+int NTHREADS = 4;
+int NSHIFT = 2;
+}
+
+namespace mi {
+
+typedef unsigned long long u64;
+
+struct Fail : std::runtime_error {
+    int code;
+    Fail(const std::string& m, int c = MI_ERR_DEVICE) : std::runtime_error(m), code(c) {}
+};
+
+#define HC(x)                                                                                   \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) throw Fail(std::string(#x) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* ensure(size_t m)
+    {
+        if (m == 0) m = 1;
+        if (m <= n) return p;
+        release();
+        HC(hipMalloc(&p, m * sizeof(T)));
+        n = m;
+        return p;
+    }
+};
+
+static hipStream_t fstream()
+{
+    static hipStream_t s = nullptr;
+    if (s == nullptr) HC(hipStreamCreate(&s));
+    return s;
+}
+
+static unsigned nblk(long long n, int b = 256) { return (unsigned)std::max<long long>(1, (n + b - 1) / b); }
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int bits_for(long long v)   // bits to hold 0..v
+{
+    int b = 1;
+    while (b < 62 && (1LL << b) <= v) b++;
+    return b;
+}
+
+template <class T>
+static T d2h1(const T* dp)
+{
+    T v;
+    HC(hipMemcpyAsync(&v, dp, sizeof(T), hipMemcpyDeviceToHost, fstream()));
+    HC(hipStreamSynchronize(fstream()));
+    return v;
+}
+
+// ---- exclusive scan: out[0..n] with out[n] = total ---------------------------------
+
+__global__ void k_scan_total(const u64* in, u64* out, long long n)
+{
+    out[n] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
+}
+
+struct Scanner {
+    std::vector<std::unique_ptr<DBuf<u64>>> lv;
+    DBuf<u64>& tmp(int i)
+    {
+        while ((int)lv.size() <= i) lv.emplace_back(new DBuf<u64>);
+        return *lv[i];
+    }
+    void run(const u64* in, u64* out, long long n, int level = 0)
+    {
+        hipStream_t st = fstream();
+        const long long nb = (n + 1023) / 1024;
+        if (n > 0) {
+            u64* sums = tmp(2 * level).ensure(nb + 1);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(scan_block<u64>), dim3(nb), dim3(256), 0, st, in, out, n, sums);
+            HC(hipGetLastError());
+            if (nb > 1) {
+                u64* ss = tmp(2 * level + 1).ensure(nb + 1);
+                run(sums, ss, nb, level + 1);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(scan_add<u64>), dim3(nb), dim3(256), 0, st, out, n, ss);
+                HC(hipGetLastError());
+            }
+        }
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_total), dim3(1), dim3(1), 0, st, in, out, n);
+        HC(hipGetLastError());
+    }
+};
+static Scanner g_scan;
+
+// ---- stable LSD radix sort over bits [lo, hi) ---------------------------------------
+
+static DBuf<u64> g_rcounts, g_roffs;
+
+template <class Rec, class KeyOf>
+static Rec* radix_sort(Rec* a, Rec* b, long long n, int lo, int hi, KeyOf key)
+{
+    if (n <= 1) return a;
+    hipStream_t st = fstream();
+    const int ntiles = (int)((n + kTile - 1) / kTile);
+    u64* counts = g_rcounts.ensure((size_t)256 * ntiles);
+    u64* offs = g_roffs.ensure((size_t)256 * ntiles + 1);
+    for (int sh = lo; sh < hi; sh += 8) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_hist<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, n, sh, key, counts, ntiles);
+        HC(hipGetLastError());
+        g_scan.run(counts, offs, (long long)256 * ntiles);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_scatter<Rec, KeyOf>), dim3(ntiles), dim3(64), 0, st, a, b, n, sh, key, offs, ntiles);
+        HC(hipGetLastError());
+        std::swap(a, b);
+    }
+    return a;
+}
+
+// ---- k-mer index -----------------------------------------------------------------------
+
+constexpr uint32_t kIdxMagic = 0x4d49584bu;   // "KXIM"
+
+struct KIdx {
+    uint32_t magic = kIdxMagic;
+    int nreads = 0, maxlen = 0, ufirst = 0, part = 0, K = 0;
+    KeyFmt fmt{1, 1};
+    long long nk = 0;
+    long long nwords = 0;
+    std::vector<int> rlen;
+    std::vector<long long> w0;
+    DBuf<uint64_t> keys, pk;
+    DBuf<long long> dw0;
+    DBuf<int> dlen;
+    // group structure (built when the index is the A side)
+    long long ng = -1;
+    DBuf<long long> gidx, gstart;
+};
+
+static int g_kmer = 14, g_binshift = 6, g_suppress = 0, g_hitmin = 35;
+
+static void layout_reads(KIdx* x, const HITS_DB* blk)
+{
+    x->nreads = blk->nreads;
+    x->maxlen = blk->maxlen;
+    x->ufirst = blk->ufirst;
+    x->part = blk->part;
+    x->rlen.resize(x->nreads);
+    x->w0.resize(x->nreads);
+    long long wpos = 1;
+    int mx = 1;
+    for (int i = 0; i < x->nreads; i++) {
+        const int L = blk->reads[i].rlen;
+        if (L < 0) throw Fail("negative read length", MI_ERR_ARG);
+        x->rlen[i] = L;
+        x->w0[i] = wpos;
+        wpos += (L + 31) / 32 + 2;
+        mx = std::max(mx, L);
+    }
+    x->nwords = wpos + 1;
+    // band arrays are sized from maxlen: never smaller than the longest read
+    x->maxlen = std::max(x->maxlen, mx);
+    x->fmt.pbits = bits_for(mx);
+    x->fmt.rbits = bits_for(std::max(1, x->nreads - 1));
+}
+
+static void build_keys(KIdx* x)
+{
+    hipStream_t st = fstream();
+    const int K = g_kmer;
+    x->K = K;
+    if (2 * K + x->fmt.rbits + x->fmt.pbits > 64)
+        throw Fail("block too large for 64-bit k-mer keys (2k + read bits + position bits > 64)", MI_ERR_ARG);
+    std::vector<long long> kofs(x->nreads + 1);
+    long long nk = 0;
+    for (int i = 0; i < x->nreads; i++) {
+        kofs[i] = nk;
+        nk += std::max(0, x->rlen[i] - K + 1);
+    }
+    kofs[x->nreads] = nk;
+    DBuf<long long> dk;
+    dk.ensure(x->nreads + 1);
+    HC(hipMemcpyAsync(dk.p, kofs.data(), sizeof(long long) * (x->nreads + 1), hipMemcpyHostToDevice, st));
+    DBuf<uint64_t> k2;
+    x->keys.ensure(nk + 1);
+    k2.ensure(nk + 1);
+    if (nk > 0) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_kmers), dim3(nblk(nk)), dim3(256), 0, st, x->pk.p, x->dw0.p, dk.p, x->nreads, nk, K, x->fmt, x->keys.p);
+        HC(hipGetLastError());
+    }
+    const int cs = x->fmt.cshift();
+    uint64_t* sorted = radix_sort(x->keys.p, k2.p, nk, cs, cs + 2 * K, KeyU64{});
+    if (sorted != x->keys.p) std::swap(x->keys.p, k2.p);   // both buffers have the same size
+    x->nk = nk;
+    if (g_suppress > 0 && nk > 0) {   // -t: drop groups of >= t entries (dalign/filter.c:645-696)
+        DBuf<u64> flag, scan;
+        DBuf<long long> gidx, gstart;
+        flag.ensure(nk);
+        scan.ensure(nk + 1);
+        gidx.ensure(nk);
+        gstart.ensure(nk + 1);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_flags), dim3(nblk(nk)), dim3(256), 0, st, x->keys.p, nk, x->fmt, flag.p);
+        g_scan.run(flag.p, scan.p, nk);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_index), dim3(nblk(nk)), dim3(256), 0, st, flag.p, scan.p, nk, gidx.p, gstart.p);
+        const long long ng = (long long)d2h1(scan.p + nk);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_set_ll), dim3(1), dim3(1), 0, st, gstart.p + ng, nk);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_keep_small), dim3(nblk(nk)), dim3(256), 0, st, gidx.p, gstart.p, nk, g_suppress, flag.p);
+        g_scan.run(flag.p, scan.p, nk);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact_u64), dim3(nblk(nk)), dim3(256), 0, st, x->keys.p, flag.p, scan.p, nk, k2.p);
+        HC(hipGetLastError());
+        std::swap(x->keys.p, k2.p);
+        x->nk = (long long)d2h1(scan.p + nk);
+    }
+    HC(hipStreamSynchronize(st));
+}
+
+static void upload_layout(KIdx* x)
+{
+    hipStream_t st = fstream();
+    x->dw0.ensure(x->nreads + 1);
+    x->dlen.ensure(x->nreads + 1);
+    x->pk.ensure(x->nwords);
+    HC(hipMemsetAsync(x->pk.p, 0, sizeof(uint64_t) * x->nwords, st));
+    if (x->nreads > 0) {
+        HC(hipMemcpyAsync(x->dw0.p, x->w0.data(), sizeof(long long) * x->nreads, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(x->dlen.p, x->rlen.data(), sizeof(int) * x->nreads, hipMemcpyHostToDevice, st));
+    }
+}
+
+static KIdx* index_from_block(HITS_DB* blk)
+{
+    std::unique_ptr<KIdx> x(new KIdx);
+    layout_reads(x.get(), blk);
+    upload_layout(x.get());
+    hipStream_t st = fstream();
+    if (x->nreads > 0) {
+        // bytes of all reads, one upload of the span they occupy
+        long long lo = INT64_MAX, hi = 0;
+        for (int i = 0; i < x->nreads; i++) {
+            lo = std::min<long long>(lo, blk->reads[i].boff);
+            hi = std::max<long long>(hi, blk->reads[i].boff + x->rlen[i]);
+        }
+        std::vector<long long> boff(x->nreads);
+        for (int i = 0; i < x->nreads; i++) boff[i] = blk->reads[i].boff - lo;
+        DBuf<uint8_t> bytes;
+        DBuf<long long> dboff;
+        bytes.ensure(hi - lo);
+        dboff.ensure(x->nreads);
+        HC(hipMemcpyAsync(bytes.p, (const char*)blk->bases + lo, hi - lo, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(dboff.p, boff.data(), sizeof(long long) * x->nreads, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pack), dim3(x->nreads), dim3(256), 0, st, bytes.p, dboff.p, x->dw0.p, x->dlen.p, x->nreads, x->pk.p);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(st));
+    }
+    build_keys(x.get());
+    return x.release();
+}
+
+static KIdx* index_complement(const KIdx* src, HITS_DB* cblk)
+{
+    if (cblk->nreads != src->nreads) throw Fail("complement block has a different read count", MI_ERR_ARG);
+    for (int i = 0; i < src->nreads; i++)
+        if (cblk->reads[i].rlen != src->rlen[i]) throw Fail("complement block has different read lengths", MI_ERR_ARG);
+    std::unique_ptr<KIdx> x(new KIdx);
+    layout_reads(x.get(), cblk);
+    upload_layout(x.get());
+    hipStream_t st = fstream();
+    if (x->nreads > 0) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_revcomp), dim3(x->nreads), dim3(256), 0, st, src->pk.p, x->pk.p, x->dw0.p, x->dlen.p, x->nreads);
+        HC(hipGetLastError());
+    }
+    build_keys(x.get());
+    return x.release();
+}
+
+static void ensure_groups(KIdx* x)
+{
+    if (x->ng >= 0) return;
+    hipStream_t st = fstream();
+    const long long n = x->nk;
+    x->gidx.ensure(n);
+    x->gstart.ensure(n + 1);
+    DBuf<u64> flag, scan;
+    flag.ensure(n);
+    scan.ensure(n + 1);
+    if (n > 0) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_flags), dim3(nblk(n)), dim3(256), 0, st, x->keys.p, n, x->fmt, flag.p);
+        g_scan.run(flag.p, scan.p, n);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_index), dim3(nblk(n)), dim3(256), 0, st, flag.p, scan.p, n, x->gidx.p, x->gstart.p);
+        HC(hipGetLastError());
+    }
+    const long long ng = n > 0 ? (long long)d2h1(scan.p + n) : 0;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_set_ll), dim3(1), dim3(1), 0, st, x->gstart.p + ng, n);
+    HC(hipStreamSynchronize(st));
+    x->ng = ng;
+}
+
+static KIdx* as_index(void* t)
+{
+    KIdx* x = (KIdx*)t;
+    if (x == nullptr || x->magic != kIdxMagic) throw Fail("not a k-mer index of this library", MI_ERR_ARG);
+    return x;
+}
+
+// ---- one Match_Filter pass ----------------------------------------------------------
+
+static MiFilterStats g_stats;
+
+struct Level {
+    int nch, pool_cap, mcap;
+    long long mtcap;
+    int waves;
+};
+
+__global__ void k_gather_start(const int* jobs, const long long* pstart, int n, long long* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pstart[jobs[i]];
+}
+
+template <int NCH>
+static void launch_report(const ReportArgs& a, int grid)
+{
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH>), dim3(grid), dim3(64), 0, fstream(), a);
+}
+
+static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, SpecObj* spec)
+{
+    MiFilterStats& S = g_stats;
+    const double t0 = now_s();
+    hipStream_t st = fstream();
+    const int K = g_kmer, BS = g_binshift;
+    S.nkmers_a = A->nk;
+    S.nkmers_b = B->nk;
+    if (A->nk == 0 || B->nk == 0) return;
+    if (A->K != K || B->K != K) throw Fail("index built with another k-mer length", MI_ERR_ARG);
+    ensure_groups(A);
+    const long long na = A->nk, nb = B->nk, ng = A->ng;
+
+    // merge-join counts (count_thread, dalign/filter.c:948-1056)
+    DBuf<long long> bs, be;
+    DBuf<u64> cnt, off, gtot, hist;
+    bs.ensure(ng);
+    be.ensure(ng);
+    cnt.ensure(na);
+    off.ensure(na + 1);
+    gtot.ensure(ng);
+    const int MAXGRAM = 10000;
+    hist.ensure(MAXGRAM);
+    HC(hipMemsetAsync(hist.p, 0, sizeof(u64) * MAXGRAM, st));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_brange), dim3(nblk(ng)), dim3(256), 0, st, A->keys.p, A->fmt, A->gstart.p, ng, B->keys.p, B->fmt, nb, bs.p, be.p);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, be.p, B->keys.p, B->fmt, self ? 1 : 0,
+                                       cnt.p);
+    g_scan.run(cnt.p, off.p, na);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_totals), dim3(nblk(ng)), dim3(256), 0, st, A->gstart.p, ng, off.p, gtot.p, hist.p, MAXGRAM);
+    HC(hipGetLastError());
+    std::vector<u64> h(MAXGRAM);
+    HC(hipMemcpyAsync(h.data(), hist.p, sizeof(u64) * MAXGRAM, hipMemcpyDeviceToHost, st));
+    const u64 total = d2h1(off.p + na);
+
+    // cap from MEM_LIMIT (dalign/filter.c:2107-2174)
+    long long limit = INT32_MAX;
+    if (MEM_LIMIT > 0) {
+        const uint64_t M16 = MEM_LIMIT / 16;
+        long long avail;
+        if (same_index || (long long)M16 > na + 2 * nb) avail = (long long)((M16 - (uint64_t)na) / 2);
+        else avail = (long long)(M16 - (uint64_t)(na + nb));
+        avail = (long long)(avail * .98);
+        long long tom = 0;
+        int j;
+        for (j = 0; j < MAXGRAM; j++) {
+            tom += (long long)j * (long long)h[j];
+            if (tom > avail) break;
+        }
+        limit = j;
+        if (limit == 1) throw Fail("insufficient memory limit (-M) for the k-mer matches", MI_ERR_ARG);
+    }
+    S.limit = (int)std::min<long long>(limit, INT32_MAX);
+    long long nh = (long long)total;
+    if (limit < INT32_MAX) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_apply_limit), dim3(nblk(na)), dim3(256), 0, st, A->gidx.p, gtot.p, na, (u64)limit, cnt.p);
+        g_scan.run(cnt.p, off.p, na);
+        nh = (long long)d2h1(off.p + na);
+        long long expect = 0;
+        for (int j = 1; j < limit && j < MAXGRAM; j++) expect += (long long)j * (long long)h[j];
+        if (expect != nh) throw Fail("seed count disagrees with the count histogram");
+    }
+    S.nhits += nh;
+    if (nh == 0) return;
+
+    // seeds, then the pair sort (dalign/filter.c:1210-1222, :2030-2054)
+    DBuf<Seed> s1, s2;
+    s1.ensure(nh + 1);
+    s2.ensure(nh + 1);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emit), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, B->keys.p, B->fmt, cnt.p, off.p, s1.p);
+    HC(hipGetLastError());
+    const int pa = A->fmt.pbits, ra = A->fmt.rbits, rb = B->fmt.rbits;
+    if (pa + ra + rb > 64) throw Fail("blocks too large for 64-bit seed keys", MI_ERR_ARG);
+    Seed* hits = radix_sort(s1.p, s2.p, nh, 0, pa + ra + rb, KeySeed{pa, pa + ra});
+    Seed* spare = (hits == s1.p) ? s2.p : s1.p;
+    (void)spare;
+
+    // read pairs
+    DBuf<u64> pflag, pscan;
+    pflag.ensure(nh);
+    pscan.ensure(nh + 1);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_flags), dim3(nblk(nh)), dim3(256), 0, st, hits, nh, pflag.p);
+    g_scan.run(pflag.p, pscan.p, nh);
+    const long long np = (long long)d2h1(pscan.p + nh);
+    DBuf<long long> pstart;
+    pstart.ensure(np + 1);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_index), dim3(nblk(nh)), dim3(256), 0, st, pflag.p, pscan.p, nh, nullptr, pstart.p);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_set_ll), dim3(1), dim3(1), 0, st, pstart.p + np, nh);
+    S.npairs += np;
+
+    // report chunks and the pairs they scan (dalign/filter.c:1699-1731, :2282-2294)
+    const int nthreads = std::max(1, NTHREADS);
+    if (nthreads > 1023 || (1 << NSHIFT) != nthreads) throw Fail("NTHREADS must be a power of 2 below 1024", MI_ERR_ARG);
+    DBuf<long long> cb;
+    cb.ensure(nthreads + 1);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_chunk_bounds), dim3(1), dim3(1024), 0, st, hits, nh, nthreads, NSHIFT, cb.p);
+    std::vector<long long> hcb(nthreads + 1);
+    HC(hipMemcpyAsync(hcb.data(), cb.p, sizeof(long long) * (nthreads + 1), hipMemcpyDeviceToHost, st));
+    const int minhit = (g_hitmin - 1) / K + 1;
+    DBuf<u64> sel, sscan;
+    sel.ensure(np);
+    sscan.ensure(np + 1);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_select), dim3(nblk(np)), dim3(256), 0, st, pstart.p, np, hits, A->dlen.p, B->dlen.p, cb.p, nthreads, minhit, HGAP_MIN,
+                                        sel.p);
+    g_scan.run(sel.p, sscan.p, np);
+    const long long njobs = (long long)d2h1(sscan.p + np);
+    S.njobs += njobs;
+    if (njobs == 0) return;
+    if (njobs > INT32_MAX) throw Fail("too many read pairs in one pass");
+    DBuf<int> jobs;
+    jobs.ensure(njobs);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact_idx), dim3(nblk(np)), dim3(256), 0, st, sel.p, sscan.p, np, jobs.p);
+    DBuf<long long> jstart;
+    jstart.ensure(njobs);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_start), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, (int)njobs, jstart.p);
+    HC(hipGetLastError());
+    std::vector<long long> hjstart(njobs);
+    HC(hipMemcpyAsync(hjstart.data(), jstart.p, sizeof(long long) * njobs, hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    const double t1 = now_s();
+    S.t_filter += t1 - t0;
+
+    // ---- report: one wave per pair ----
+    DevSpecHost* ds = mi_spec_device(&spec->prm);
+    if (ds == nullptr) throw Fail("cannot upload alignment tables");
+    const int TS = spec->hs.tspace;
+    const int maxdiag = A->maxlen >> BS, mindiag = (-B->maxlen) >> BS;
+    const int w = maxdiag - mindiag + 1;
+    const int maxtp_all = 2 * (std::max(A->maxlen, B->maxlen) / TS + 2);
+    const int cus = mi_num_cus();
+    const Level levels[4] = {{1, 1 << 17, 256, 1 << 18, cus * 8},
+                             {4, 1 << 17, 256, 1 << 18, cus * 8},
+                             {16, 1 << 17, 256, 1 << 18, cus * 4},
+                             {16, 1 << 20, 4096, 1LL << 22, std::min(cus, 256)}};
+    const int small_trace = TS <= TRACE_XOVR ? 1 : 0;
+    const int tbytes = small_trace ? 1 : 2;
+
+    DBuf<int> bands, status, edge_in, edge_out, touch, order, counter;
+    DBuf<Peb> pools, pts;
+    DBuf<uint16_t> latr, mtrace;
+    DBuf<MPath> mpaths;
+    DBuf<u64> ocount, stats;
+    status.ensure(njobs);
+    edge_in.ensure(njobs);
+    edge_out.ensure(njobs);
+    touch.ensure(njobs);
+    order.ensure(njobs);
+    counter.ensure(1);
+    ocount.ensure(2);
+    stats.ensure(4);
+    HC(hipMemsetAsync(ocount.p, 0, 2 * sizeof(u64), st));
+    HC(hipMemsetAsync(stats.p, 0, 4 * sizeof(u64), st));
+    HC(hipMemsetAsync(edge_in.p, 0, sizeof(int) * njobs, st));
+
+    long long ocap = std::max<long long>(4096, 2 * njobs);
+    long long otcap = ocap * 160;
+    DBuf<OutRec> orecs;
+    DBuf<uint8_t> otr;
+    orecs.ensure(ocap);
+    otr.ensure(otcap);
+    HC(hipMemsetAsync(orecs.p, 0xff, sizeof(OutRec) * ocap, st));
+
+    ReportArgs a;
+    memset(&a, 0, sizeof a);
+    a.hits = hits;
+    a.pstart = pstart.p;
+    a.jobs = jobs.p;
+    a.pkA = A->pk.p;
+    a.w0A = A->dw0.p;
+    a.lenA = A->dlen.p;
+    a.pkB = B->pk.p;
+    a.w0B = B->dw0.p;
+    a.lenB = B->dlen.p;
+    a.afirst = A->ufirst;
+    a.bfirst = B->ufirst;
+    a.comp = comp;
+    a.self = self ? 1 : 0;
+    a.symmetric = SYMMETRIC;
+    a.hgap_min = HGAP_MIN;
+    a.K = K;
+    a.binshift = BS;
+    a.hitmin = g_hitmin;
+    a.minhit = minhit;
+    a.minover = MINOVER;
+    a.tspace = TS;
+    a.small_trace = small_trace;
+    a.mindiag = mindiag;
+    a.maxdiag = maxdiag;
+    a.w = w;
+    a.sp = ds->spec;
+    a.maxtp_all = maxtp_all;
+    a.edge_out = edge_out.p;
+    a.touch = touch.p;
+    a.status = status.p;
+    a.counter = counter.p;
+    a.abort_flag = mi_abort_flag_device();
+    a.stats = stats.p;
+    a.ocount = ocount.p;
+
+    std::vector<int> hstatus(njobs, -1), final_launch(njobs, -1), hedge(njobs, 0), htouch(njobs, 0);
+    std::vector<int> hedge_in(njobs, 0);
+    std::vector<std::pair<long long, long long>> launches;   // record index range of every launch
+    int cur_level = -1;
+
+    // runs `todo` at `level`; returns the jobs that need another launch
+    auto run = [&](std::vector<int>& todo, int level, bool with_edge_in) {
+        const Level& L = levels[level];
+        const int grid = (int)std::min<long long>(L.waves, (long long)todo.size());
+        if (level != cur_level) {
+            bands.release();
+            pools.release();
+            pts.release();
+            latr.release();
+            mpaths.release();
+            mtrace.release();
+            cur_level = level;
+        }
+        const int gmax = L.waves;
+        bands.ensure((size_t)(3 * w + 2) * gmax);
+        pools.ensure((size_t)L.pool_cap * gmax);
+        pts.ensure((size_t)2 * 8192 * gmax);
+        latr.ensure((size_t)4 * maxtp_all * gmax);
+        mpaths.ensure((size_t)2 * L.mcap * gmax);
+        mtrace.ensure((size_t)L.mtcap * gmax);
+        HC(hipMemsetAsync(bands.p, 0, sizeof(int) * (size_t)(3 * w + 2) * gmax, st));
+        HC(hipMemsetAsync(counter.p, 0, sizeof(int), st));
+        HC(hipMemcpyAsync(order.p, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice, st));
+        if (with_edge_in)
+            HC(hipMemcpyAsync(edge_in.p, hedge_in.data(), sizeof(int) * njobs, hipMemcpyHostToDevice, st));
+        u64 oc[2];
+        HC(hipMemcpyAsync(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost, st));
+        HC(hipStreamSynchronize(st));
+        ReportArgs b = a;
+        b.order = order.p;
+        b.norder = (int)todo.size();
+        b.bands = bands.p;
+        b.pools = pools.p;
+        b.pool_cap = L.pool_cap;
+        b.pts = pts.p;
+        b.pts_cap = 8192;
+        b.latr = latr.p;
+        b.mpaths = mpaths.p;
+        b.mcap = L.mcap;
+        b.mtrace = mtrace.p;
+        b.mtcap = L.mtcap;
+        b.edge_in = edge_in.p;
+        b.orecs = orecs.p;
+        b.ocap = ocap;
+        b.otr = otr.p;
+        b.otcap = otcap;
+        switch (L.nch) {
+        case 1: launch_report<1>(b, grid); break;
+        case 4: launch_report<4>(b, grid); break;
+        default: launch_report<16>(b, grid); break;
+        }
+        HC(hipGetLastError());
+        const int wd = mi_wait_watchdog(st, "Match_Filter report kernel");
+        if (wd > 0) throw Fail("report kernel exceeded the watchdog deadline and was stopped", MI_ERR_TIMEOUT);
+        if (wd < 0) throw Fail("report kernel failed on the device");
+        u64 oc2[2];
+        HC(hipMemcpy(oc2, ocount.p, sizeof oc2, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(hstatus.data(), status.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(hedge.data(), edge_out.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(htouch.data(), touch.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
+        const int li = (int)launches.size();
+        launches.push_back({(long long)oc[0], (long long)oc2[0]});
+        std::vector<int> again;
+        bool grow = false;
+        for (int j : todo) {
+            const int s = hstatus[j];
+            if (s == RP_OK) final_launch[j] = li;
+            else if (s == RP_UNDEF) {
+                char m[160];
+                snprintf(m, sizeof m, "pair job %d: an alignment reached a wave state the reference leaves undefined", j);
+                throw Fail(m, MI_ERR_UNDEF);
+            } else {
+                again.push_back(j);
+                if (s == RP_OUT) grow = true;
+            }
+        }
+        if (grow || (long long)oc2[0] > ocap || (long long)oc2[1] > otcap) {
+            // grow the output arenas past what has been reserved so far
+            const long long nrec = std::max<long long>(ocap * 2, (long long)oc2[0] * 3 / 2 + 4096);
+            const long long ntr = std::max<long long>(otcap * 2, (long long)oc2[1] * 3 / 2 + 65536);
+            DBuf<OutRec> r2;
+            DBuf<uint8_t> t2;
+            r2.ensure(nrec);
+            t2.ensure(ntr);
+            HC(hipMemsetAsync(r2.p, 0xff, sizeof(OutRec) * nrec, st));
+            HC(hipMemcpyAsync(r2.p, orecs.p, sizeof(OutRec) * ocap, hipMemcpyDeviceToDevice, st));
+            HC(hipMemcpyAsync(t2.p, otr.p, otcap, hipMemcpyDeviceToDevice, st));
+            HC(hipStreamSynchronize(st));
+            std::swap(orecs.p, r2.p);
+            std::swap(orecs.n, r2.n);
+            std::swap(otr.p, t2.p);
+            std::swap(otr.n, t2.n);
+            ocap = nrec;
+            otcap = ntr;
+        }
+        return again;
+    };
+
+    // runs `todo` until every job is done: output-arena overflows are re-run
+    // at the same window, wide bands / full pools at the next one
+    auto run_all = [&](std::vector<int> todo, bool with_edge_in) {
+        int level = 0;
+        while (!todo.empty()) {
+            std::vector<int> again = run(todo, level, with_edge_in);
+            S.retries += (long long)again.size();
+            bool up = false, arena = false;
+            for (int j : again) {
+                up |= hstatus[j] != RP_OUT;
+                arena |= hstatus[j] == RP_ARENA;
+            }
+            if (up) {
+                if (level == 3) throw Fail("pairs exceed the largest aligner window / arena");
+                level = arena ? 3 : level + 1;
+            }
+            todo.swap(again);
+        }
+    };
+    {
+        std::vector<int> todo(njobs);
+        for (long long j = 0; j < njobs; j++) todo[j] = (int)j;
+        run_all(todo, false);
+    }
+
+    // the cell shared by lastp[maxdiag] and lasta[mindiag-1] carries from
+    // one pair to the next within a report chunk: re-run the pairs that
+    // would have seen a non-zero value (rare; needs an alignment reaching
+    // the lowest band)
+    {
+        std::vector<int> job_chunk(njobs);
+        for (long long j = 0; j < njobs; j++) {
+            int c = 0;
+            while (c + 1 < nthreads && hcb[c + 1] <= hjstart[j]) c++;
+            job_chunk[j] = c;
+        }
+        std::vector<long long> cstart(nthreads + 1, njobs);
+        for (long long j = njobs - 1; j >= 0; j--) cstart[job_chunk[j]] = j;
+        for (int c = nthreads - 1; c >= 0; c--)
+            if (cstart[c] > cstart[c + 1]) cstart[c] = cstart[c + 1];
+        std::vector<long long> pos(nthreads);
+        std::vector<int> carry(nthreads, 0);
+        for (int c = 0; c < nthreads; c++) pos[c] = cstart[c];
+        for (;;) {
+            std::vector<int> rerun;
+            for (int c = 0; c < nthreads; c++) {
+                long long j = pos[c];
+                const long long end = cstart[c + 1];
+                for (; j < end; j++) {
+                    if (!htouch[j]) continue;
+                    if (hedge_in[j] != carry[c]) {
+                        hedge_in[j] = carry[c];
+                        rerun.push_back((int)j);
+                        break;
+                    }
+                    carry[c] = hedge[j];
+                }
+                pos[c] = j;
+            }
+            if (rerun.empty()) break;
+            S.edge_fixups += (long long)rerun.size();
+            run_all(rerun, true);
+        }
+    }
+    const double t2 = now_s();
+    S.t_report += t2 - t1;
+
+    // ---- records back to the host ----
+    u64 oc[2], sv[4];
+    HC(hipMemcpy(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(sv, stats.p, sizeof sv, hipMemcpyDeviceToHost));
+    S.naligned += (long long)sv[0];
+    S.cells += (long long)sv[1];
+    S.snake += (long long)sv[2];
+    S.edge_events += (long long)sv[3];
+    const long long nrec = std::min<long long>((long long)oc[0], ocap);
+    std::vector<OutRec> recs(nrec);
+    if (nrec > 0) HC(hipMemcpy(recs.data(), orecs.p, sizeof(OutRec) * nrec, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> tr(std::min<long long>((long long)oc[1], otcap));
+    if (!tr.empty()) HC(hipMemcpy(tr.data(), otr.p, tr.size(), hipMemcpyDeviceToHost));
+    std::vector<int> keep;
+    keep.reserve(nrec);
+    for (long long i = 0; i < nrec; i++) {
+        const OutRec& r = recs[i];
+        if (r.job < 0 || r.job >= njobs) continue;
+        const int L = final_launch[r.job];
+        if (L < 0 || i < launches[L].first || i >= launches[L].second) continue;
+        keep.push_back((int)i);
+    }
+    // order of equal (aread, bread, COMP, abpos) keys: the pair's list order
+    std::sort(keep.begin(), keep.end(), [&](int x, int y) {
+        const OutRec& l = recs[x];
+        const OutRec& r = recs[y];
+        if (l.aread != r.aread) return l.aread < r.aread;
+        if (l.bread != r.bread) return l.bread < r.bread;
+        if (l.abpos != r.abpos) return l.abpos < r.abpos;
+        return l.local < r.local;
+    });
+    Overlap_IO_Buffer* ob = spec->iobuf;
+    for (int i : keep) {
+        const OutRec& r = recs[i];
+        Overlap o;
+        o.path.tlen = r.tlen;
+        o.path.diffs = r.diffs;
+        o.path.abpos = r.abpos;
+        o.path.bbpos = r.bbpos;
+        o.path.aepos = r.aepos;
+        o.path.bepos = r.bepos;
+        o.path.trace = tr.data() + r.toff;
+        o.flags = r.flags;
+        o.aread = r.aread;
+        o.bread = r.bread;
+        if (AddOverlapToBuffer(ob, &o, tbytes)) throw Fail("cannot grow the overlap buffer");
+    }
+    S.nrecords += (long long)keep.size();
+    S.t_copy += now_s() - t2;
+}
+
+}  // namespace mi
+
+using namespace mi;
+
+// ---- C-ABI ------------------------------------------------------------------------------
+
+extern "C" {
+
+int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin)
+{
+    if (kmer <= 1) return 1;
+    if (kmer > 14) {
+        mi_set_error("k-mer length above 14 is not supported by the 64-bit k-mer keys");
+        return 1;
+    }
+    g_kmer = kmer;
+    g_binshift = binshift;
+    g_suppress = suppress;
+    g_hitmin = hitmin;
+    return 0;
+}
+
+static void* sort_kmers_impl(HITS_DB* block, int* len, const KIdx* comp_of)
+{
+    try {
+        if (BIASED) throw Fail("biased k-mer selection (-b) is not supported", MI_ERR_ARG);
+        KIdx* x = comp_of ? index_complement(comp_of, block) : index_from_block(block);
+        if (len) *len = (int)std::min<long long>(x->nk, INT32_MAX);
+        return x;
+    } catch (const Fail& e) {
+        mi_set_error(e.what());
+        return nullptr;
+    } catch (const std::bad_alloc&) {
+        mi_set_error("host out of memory");
+        return nullptr;
+    }
+}
+
+void* Sort_Kmers(HITS_DB* block, int* len)
+{
+    void* r = sort_kmers_impl(block, len, nullptr);
+    if (r == nullptr) {
+        fprintf(stderr, "Sort_Kmers: %s\n", mi_last_error());
+        exit(1);
+    }
+    return r;
+}
+
+void* mi_sort_kmers_complement(void* table, HITS_DB* cblock, int* len)
+{
+    try {
+        return sort_kmers_impl(cblock, len, as_index(table));
+    } catch (const Fail& e) {
+        mi_set_error(e.what());
+        return nullptr;
+    }
+}
+
+void mi_free_kmer_index(void* table)
+{
+    KIdx* x = (KIdx*)table;
+    if (x && x->magic == kIdxMagic) {
+        x->magic = 0;
+        delete x;
+    }
+}
+
+int mi_match_filter(char* aname, HITS_DB* ablock, char* bname, HITS_DB* bblock, void* atable, void* btable, int comp,
+                    Align_Spec* asettings)
+{
+    (void)ablock;
+    (void)bblock;
+    memset(&g_stats, 0, sizeof g_stats);
+    const double t0 = now_s();
+    int rc = 0;
+    try {
+        KIdx* A = as_index(atable);
+        KIdx* B = as_index(btable);
+        const bool self = (aname == bname);
+        if (IDENTITY) throw Fail("identity mode (-I) is not supported", MI_ERR_ARG);
+        match_pass(A, B, self, comp ? 1 : 0, atable == btable, (SpecObj*)asettings);
+    } catch (const Fail& e) {
+        mi_set_error(e.what());
+        rc = e.code;
+    } catch (const std::bad_alloc&) {
+        mi_set_error("host out of memory");
+        rc = MI_ERR_DEVICE;
+    }
+    // the B index is consumed when it is not the A index (dalign/filter.c:2194-2202)
+    if (btable != atable) mi_free_kmer_index(btable);
+    g_stats.t_total = now_s() - t0;
+    return rc;
+}
+
+void Match_Filter(char* aname, HITS_DB* ablock, char* bname, HITS_DB* bblock, void* atable, int alen, void* btable,
+                  int blen, int comp, Align_Spec* asettings)
+{
+    (void)alen;
+    (void)blen;
+    if (mi_match_filter(aname, ablock, bname, bblock, atable, btable, comp, asettings) != 0) {
+        fprintf(stderr, "Match_Filter: %s\n", mi_last_error());
+        exit(1);
+    }
+    if (VERBOSE) {
+        const MiFilterStats& s = g_stats;
+        printf("\n     %lld %d-mers, %lld seed hits, %lld confirmed hits (%.3f s)\n", s.nhits, g_kmer, s.naligned,
+               s.nrecords, s.t_total);
+        fflush(stdout);
+    }
+}
+
+void mi_filter_stats(MiFilterStats* out) { *out = g_stats; }
+
+}  // extern "C"

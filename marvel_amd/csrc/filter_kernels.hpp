@@ -3,9 +3,10 @@
 // seed pairs, the seed sort and the split of the seeds into read pairs.
 //
 // Key of an index entry (dalign/filter.c:69-74 KmerPos, restated):
-//   key = code << 36 | read << 18 | rpos       (code = 2K bits, K <= 14)
-// so sorting keys orders by (code, read, rpos), the order Sort_Kmers'
-// stable byte sort leaves (dalign/filter.c:698-903).
+//   key = code << (rbits + pbits) | read << pbits | rpos
+// with rbits / pbits the bit widths of the block's read count and longest
+// read, so sorting keys orders by (code, read, rpos), the order Sort_Kmers'
+// stable byte sort leaves (dalign/filter.c:698-903).  2K + rbits + pbits <= 64.
 // Seed record (dalign/filter.c:76-82 SeedPair): diag, apos, aread, bread.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -13,9 +14,13 @@
 
 namespace mi {
 
-constexpr int kReadBits = 18;
-constexpr int kPosBits = 18;
-constexpr int kCodeShift = kReadBits + kPosBits;   // 36
+struct KeyFmt {
+    int rbits, pbits;
+    __device__ __host__ __forceinline__ int cshift() const { return rbits + pbits; }
+    __device__ __forceinline__ uint64_t code(uint64_t k) const { return k >> cshift(); }
+    __device__ __forceinline__ int read(uint64_t k) const { return (int)((k >> pbits) & ((1ull << rbits) - 1)); }
+    __device__ __forceinline__ int pos(uint64_t k) const { return (int)(k & ((1ull << pbits) - 1)); }
+};
 
 struct Seed {
     int diag, apos, aread, bread;
@@ -26,17 +31,65 @@ struct KeyU64 {
 };
 
 // (bread, aread, apos) key of a seed, the order of the pair sort
-// (dalign/filter.c:2030-2054: LSD on apos, aread, bread bytes).
+// (dalign/filter.c:2030-2054: stable LSD on apos, aread, bread bytes)
 struct KeySeed {
+    int sa, sb;   // aread shift (= apos bits), bread shift (= apos + aread bits)
     __device__ __forceinline__ uint64_t operator()(const Seed& s) const
     {
-        return ((uint64_t)(uint32_t)s.bread << 36) | ((uint64_t)(uint32_t)s.aread << 18) | (uint32_t)s.apos;
+        return ((uint64_t)(uint32_t)s.bread << sb) | ((uint64_t)(uint32_t)s.aread << sa) | (uint32_t)s.apos;
     }
 };
 
-// one thread per k-mer: idx -> (read, rpos) -> key
-__global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __restrict__ word0,
-                        const long long* __restrict__ kofs, int nreads, long long nk, int K,
+// 2-bit packing of a block: read r's bases (values 0..3, one byte each) at
+// bases[boff[r]] go to words w0[r] .. ; base i at bits 2*(i&31) of word i>>5
+__global__ void k_pack(const uint8_t* __restrict__ bases, const long long* __restrict__ boff,
+                       const long long* __restrict__ w0, const int* __restrict__ rlen, int nreads,
+                       uint64_t* __restrict__ pk)
+{
+    const int r = blockIdx.x;
+    if (r >= nreads) return;
+    const int len = rlen[r];
+    const uint8_t* s = bases + boff[r];
+    uint64_t* d = pk + w0[r];
+    const int nw = (len + 31) >> 5;
+    for (int wi = threadIdx.x; wi < nw; wi += blockDim.x) {
+        uint64_t v = 0;
+        const int j0 = wi * 32;
+        const int m = min(32, len - j0);
+        for (int b = 0; b < m; b++) v |= (uint64_t)(s[j0 + b] & 3) << (2 * b);
+        d[wi] = v;
+    }
+}
+
+// reverse complement of every read (dalign/daligner.c:456-573): base j of
+// the output read = 3 - base (len-1-j) of the input read
+__global__ void k_revcomp(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                          const long long* __restrict__ w0, const int* __restrict__ rlen, int nreads)
+{
+    const int r = blockIdx.x;
+    if (r >= nreads) return;
+    const int len = rlen[r];
+    const uint64_t* s = in + w0[r];
+    uint64_t* d = out + w0[r];
+    const int nw = (len + 31) >> 5;
+    for (int wi = threadIdx.x; wi < nw; wi += blockDim.x) {
+        uint64_t v = 0;
+        const int j0 = wi * 32;
+        const int m = min(32, len - j0);
+        for (int b = 0; b < m; b++) {
+            const int q = len - 1 - (j0 + b);
+            const uint64_t base = (s[q >> 5] >> (2 * (q & 31))) & 3;
+            v |= (3 - base) << (2 * b);
+        }
+        d[wi] = v;
+    }
+}
+
+// one thread per k-mer: index -> (read, rpos) -> key.  kofs[r] = k-mers
+// before read r; rpos is the position of the k-mer's last base
+// (dalign/filter.c:400-490).
+__global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __restrict__ w0,
+                        const long long* __restrict__ kofs, int nreads, long long nk, int K, KeyFmt fmt,
                         uint64_t* __restrict__ keys)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,101 +101,101 @@ __global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __rest
         else hi = m;
     }
     const int r = lo;
-    const int rpos = (int)(i - kofs[r]) + K - 1;   // position of the k-mer's last base
-    const uint64_t* w = pk + word0[r];
+    const int rpos = (int)(i - kofs[r]) + K - 1;
+    const uint64_t* w = pk + w0[r];
+    // bases rpos-K+1 .. rpos, first base most significant
+    const int b0 = rpos - K + 1;
+    const int wi = b0 >> 5, sh = 2 * (b0 & 31);
+    // 2K <= 28 bits span at most two words
+    uint64_t win = w[wi] >> sh;
+    if (sh) win |= w[wi + 1] << (64 - sh);
     uint64_t code = 0;
-    for (int j = rpos - K + 1; j <= rpos; j++) code = (code << 2) | ((w[j >> 5] >> (2 * (j & 31))) & 3);
-    keys[i] = (code << kCodeShift) | ((uint64_t)r << kPosBits) | (uint64_t)rpos;
-}
-
-// reverse complement of every read (dalign/daligner.c:456-573): base j of
-// the output read = 3 - base (len-1-j) of the input read
-__global__ void k_revcomp(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
-                          const long long* __restrict__ word0, const int* __restrict__ rlen, int nreads)
-{
-    const int r = blockIdx.x;
-    if (r >= nreads) return;
-    const int len = rlen[r];
-    const uint64_t* s = in + word0[r];
-    uint64_t* d = out + word0[r];
-    const int nw = (len + 31) >> 5;
-    for (int wi = threadIdx.x; wi < nw; wi += blockDim.x) {
-        uint64_t v = 0;
-        for (int b = 0; b < 32; b++) {
-            const int j = wi * 32 + b;
-            if (j >= len) break;
-            const int q = len - 1 - j;
-            const uint64_t base = (s[q >> 5] >> (2 * (q & 31))) & 3;
-            v |= (3 - base) << (2 * b);
-        }
-        d[wi] = v;
-    }
+#pragma unroll 4
+    for (int j = 0; j < K; j++) code = (code << 2) | ((win >> (2 * j)) & 3);
+    keys[i] = (code << fmt.cshift()) | ((uint64_t)r << fmt.pbits) | (uint64_t)rpos;
 }
 
 // group starts of a sorted key list (by code)
-__global__ void k_group_flags(const uint64_t* __restrict__ keys, long long n, unsigned long long* __restrict__ flag)
+__global__ void k_group_flags(const uint64_t* __restrict__ keys, long long n, KeyFmt fmt,
+                              unsigned long long* __restrict__ flag)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    flag[i] = (i == 0 || (keys[i] >> kCodeShift) != (keys[i - 1] >> kCodeShift)) ? 1 : 0;
+    flag[i] = (i == 0 || fmt.code(keys[i]) != fmt.code(keys[i - 1])) ? 1 : 0;
 }
 
-// group index of every entry: inclusive count of group starts, minus one
+// inclusive count of set flags minus one: the group of every entry; and
+// the start of every group
 __global__ void k_group_index(const unsigned long long* __restrict__ flag, const unsigned long long* __restrict__ excl,
-                              long long n, long long* __restrict__ gidx)
+                              long long n, long long* __restrict__ gidx, long long* __restrict__ gstart)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    gidx[i] = (long long)(excl[i] + flag[i]) - 1;
+    const long long g = (long long)(excl[i] + flag[i]) - 1;
+    if (gidx) gidx[i] = g;
+    if (flag[i]) gstart[g] = i;
 }
 
-__global__ void k_group_starts(const unsigned long long* __restrict__ flag, const long long* __restrict__ gidx,
-                               long long n, long long* __restrict__ gstart)
+__global__ void k_set_ll(long long* __restrict__ p, long long v) { *p = v; }
+
+// -t suppression (dalign/filter.c:645-696): entries of groups of size >= t are dropped
+__global__ void k_keep_small(const long long* __restrict__ gidx, const long long* __restrict__ gstart, long long n,
+                             long long t, unsigned long long* __restrict__ keep)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (flag[i]) gstart[gidx[i]] = i;
+    const long long g = gidx[i];
+    keep[i] = (gstart[g + 1] - gstart[g] < t) ? 1 : 0;
 }
 
-__device__ __forceinline__ long long lower_code(const uint64_t* keys, long long n, uint64_t code)
+__global__ void k_compact_u64(const uint64_t* __restrict__ in, const unsigned long long* __restrict__ keep,
+                              const unsigned long long* __restrict__ pos, long long n, uint64_t* __restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (keep[i]) out[pos[i]] = in[i];
+}
+
+__device__ __forceinline__ long long lower_code(const uint64_t* keys, long long n, uint64_t code, const KeyFmt& f)
 {
     long long lo = 0, hi = n;
     while (lo < hi) {
         const long long m = (lo + hi) >> 1;
-        if ((keys[m] >> kCodeShift) < code) lo = m + 1;
+        if (f.code(keys[m]) < code) lo = m + 1;
         else hi = m;
     }
     return lo;
 }
 
-// per A group: the matching B range
-__global__ void k_group_brange(const uint64_t* __restrict__ ak, const long long* __restrict__ gstart, long long ng,
-                               const uint64_t* __restrict__ bk, long long nb, long long* __restrict__ bs,
-                               long long* __restrict__ be)
+// per A group: the B range with the same code (find_tuple, dalign/filter.c:911-930)
+__global__ void k_group_brange(const uint64_t* __restrict__ ak, KeyFmt fa, const long long* __restrict__ gstart,
+                               long long ng, const uint64_t* __restrict__ bk, KeyFmt fb, long long nb,
+                               long long* __restrict__ bs, long long* __restrict__ be)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
-    const uint64_t code = ak[gstart[g]] >> kCodeShift;
-    bs[g] = lower_code(bk, nb, code);
-    be[g] = lower_code(bk, nb, code + 1);
+    const uint64_t code = fa.code(ak[gstart[g]]);
+    bs[g] = lower_code(bk, nb, code, fb);
+    be[g] = lower_code(bk, nb, code + 1, fb);
 }
 
-// per A entry: number of B entries it pairs with (dalign/filter.c:948-1056):
-// self mode pairs only with B reads below the A read, otherwise the whole group
-__global__ void k_counts(const uint64_t* __restrict__ ak, long long na, const long long* __restrict__ gidx,
+// per A entry: number of B entries it pairs with (count_thread,
+// dalign/filter.c:948-1056): in self mode only B reads below the A read,
+// otherwise the whole B group
+__global__ void k_counts(const uint64_t* __restrict__ ak, KeyFmt fa, long long na, const long long* __restrict__ gidx,
                          const long long* __restrict__ bs, const long long* __restrict__ be,
-                         const uint64_t* __restrict__ bk, int self, unsigned long long* __restrict__ cnt)
+                         const uint64_t* __restrict__ bk, KeyFmt fb, int self, unsigned long long* __restrict__ cnt)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= na) return;
     const long long g = gidx[i];
     long long s = bs[g], e = be[g];
     if (self) {
-        const uint64_t ar = (ak[i] >> kPosBits) & ((1ull << kReadBits) - 1);
+        const int ar = fa.read(ak[i]);
         long long lo = s, hi = e;
         while (lo < hi) {
             const long long m = (lo + hi) >> 1;
-            if (((bk[m] >> kPosBits) & ((1ull << kReadBits) - 1)) < ar) lo = m + 1;
+            if (fb.read(bk[m]) < ar) lo = m + 1;
             else hi = m;
         }
         e = lo;
@@ -150,20 +203,19 @@ __global__ void k_counts(const uint64_t* __restrict__ ak, long long na, const lo
     cnt[i] = (unsigned long long)(e - s);
 }
 
-// group totals (from the exclusive scan of counts) and their histogram
-__global__ void k_group_totals(const long long* __restrict__ gstart, long long ng, long long na,
-                               const unsigned long long* __restrict__ cscan, const unsigned long long* __restrict__ cnt,
-                               unsigned long long* __restrict__ gtot, unsigned long long* __restrict__ hist, int maxgram)
+// group totals (from the exclusive scan of counts) and their histogram (hitgram)
+__global__ void k_group_totals(const long long* __restrict__ gstart, long long ng,
+                               const unsigned long long* __restrict__ cscan, unsigned long long* __restrict__ gtot,
+                               unsigned long long* __restrict__ hist, int maxgram)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
-    const long long s = gstart[g], e = (g + 1 < ng) ? gstart[g + 1] : na;
-    const unsigned long long t = cscan[e - 1] + cnt[e - 1] - cscan[s];
+    const unsigned long long t = cscan[gstart[g + 1]] - cscan[gstart[g]];
     gtot[g] = t;
     if (t < (unsigned long long)maxgram) atomicAdd(&hist[t], 1ull);
 }
 
-// counts of entries whose group reaches the cap are dropped (dalign/filter.c:1148-1154)
+// groups whose total reaches the cap are dropped (merge_thread, dalign/filter.c:1148-1154)
 __global__ void k_apply_limit(const long long* __restrict__ gidx, const unsigned long long* __restrict__ gtot,
                               long long na, unsigned long long limit, unsigned long long* __restrict__ cnt)
 {
@@ -172,9 +224,10 @@ __global__ void k_apply_limit(const long long* __restrict__ gidx, const unsigned
     if (gtot[gidx[i]] >= limit) cnt[i] = 0;
 }
 
-// seeds in merge order (dalign/filter.c:1210-1222)
-__global__ void k_emit(const uint64_t* __restrict__ ak, long long na, const long long* __restrict__ gidx,
-                       const long long* __restrict__ bs, const uint64_t* __restrict__ bk,
+// seeds in merge order (dalign/filter.c:1210-1222): for every A entry its
+// B partners in index order
+__global__ void k_emit(const uint64_t* __restrict__ ak, KeyFmt fa, long long na, const long long* __restrict__ gidx,
+                       const long long* __restrict__ bs, const uint64_t* __restrict__ bk, KeyFmt fb,
                        const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ off,
                        Seed* __restrict__ out)
 {
@@ -183,15 +236,12 @@ __global__ void k_emit(const uint64_t* __restrict__ ak, long long na, const long
     const unsigned long long c = cnt[i];
     if (c == 0) return;
     const uint64_t a = ak[i];
-    const int ar = (int)((a >> kPosBits) & ((1ull << kReadBits) - 1));
-    const int ap = (int)(a & ((1ull << kPosBits) - 1));
+    const int ar = fa.read(a), ap = fa.pos(a);
     const long long s = bs[gidx[i]];
     Seed* o = out + off[i];
     for (unsigned long long j = 0; j < c; j++) {
         const uint64_t b = bk[s + j];
-        const int br = (int)((b >> kPosBits) & ((1ull << kReadBits) - 1));
-        const int bp = (int)(b & ((1ull << kPosBits) - 1));
-        o[j] = Seed{ap - bp, ap, ar, br};
+        o[j] = Seed{ap - fb.pos(b), ap, ar, fb.read(b)};
     }
 }
 
@@ -203,12 +253,55 @@ __global__ void k_pair_flags(const Seed* __restrict__ h, long long n, unsigned l
     flag[i] = (i == 0 || h[i].aread != h[i - 1].aread || h[i].bread != h[i - 1].bread) ? 1 : 0;
 }
 
-__global__ void k_pair_starts(const unsigned long long* __restrict__ flag, const long long* __restrict__ pidx,
-                              long long n, long long* __restrict__ pstart)
+// report_thread chunk starts (dalign/filter.c:2282-2294): p = (n*i) >> nshift
+// moved past the run of seeds with the B read of seed p-1
+__global__ void k_chunk_bounds(const Seed* __restrict__ h, long long n, int nthreads, int nshift,
+                               long long* __restrict__ cb)
+{
+    const int i = threadIdx.x;
+    if (i > nthreads) return;
+    if (i == 0) { cb[0] = 0; return; }
+    if (i == nthreads) { cb[i] = n; return; }
+    long long p = (long long)(((unsigned long long)n * (unsigned long long)i) >> nshift);
+    if (p > 0 && p < n) {
+        const int d = h[p - 1].bread;
+        long long lo = p, hi = n;   // first index >= p whose B read differs (seeds are sorted by B read)
+        while (lo < hi) {
+            const long long m = (lo + hi) >> 1;
+            if (h[m].bread == d) lo = m + 1;
+            else hi = m;
+        }
+        p = lo;
+    }
+    cb[i] = p;
+}
+
+// pairs report_thread aligns (dalign/filter.c:1699-1731): at least minhit
+// seeds, starting before the chunk's end - minhit, and not both reads
+// shorter than the -H cutoff
+__global__ void k_select(const long long* __restrict__ pstart, long long np, const Seed* __restrict__ h,
+                         const int* __restrict__ lenA, const int* __restrict__ lenB, const long long* __restrict__ cb,
+                         int nthreads, int minhit, int hgap, unsigned long long* __restrict__ sel)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const long long s = pstart[p], e = pstart[p + 1];
+    int c = 0;   // chunk holding s: last c with cb[c] <= s
+    while (c + 1 < nthreads && cb[c + 1] <= s) c++;
+    bool ok = (e - s >= minhit) && (s < cb[c + 1] - minhit);
+    if (ok) {
+        const Seed x = h[s];
+        if (lenA[x.aread] < hgap && lenB[x.bread] < hgap) ok = false;
+    }
+    sel[p] = ok ? 1 : 0;
+}
+
+__global__ void k_compact_idx(const unsigned long long* __restrict__ sel, const unsigned long long* __restrict__ pos,
+                              long long n, int* __restrict__ out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (flag[i]) pstart[pidx[i]] = i;
+    if (sel[i]) out[pos[i]] = (int)i;
 }
 
 }  // namespace mi

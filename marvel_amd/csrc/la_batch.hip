@@ -84,13 +84,13 @@ static int launch_la(int nch, int grid, const LaTask* tasks, const int* order, i
     if (hipMemsetAsync(counter, 0, sizeof(int), st) != hipSuccess) return -1;
     switch (nch) {
     case 1:
-        la_batch_kernel<1><<<grid, 64, 0, st>>>(tasks, order, n, pk, sp, trbuf, outs, pools, pool_cap, pts, pts_cap, counter, abort_flag);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(la_batch_kernel<1>), dim3(grid), dim3(64), 0, st, tasks, order, n, pk, sp, trbuf, outs, pools, pool_cap, pts, pts_cap, counter, abort_flag);
         break;
     case 4:
-        la_batch_kernel<4><<<grid, 64, 0, st>>>(tasks, order, n, pk, sp, trbuf, outs, pools, pool_cap, pts, pts_cap, counter, abort_flag);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(la_batch_kernel<4>), dim3(grid), dim3(64), 0, st, tasks, order, n, pk, sp, trbuf, outs, pools, pool_cap, pts, pts_cap, counter, abort_flag);
         break;
     case 16:
-        la_batch_kernel<16><<<grid, 64, 0, st>>>(tasks, order, n, pk, sp, trbuf, outs, pools, pool_cap, pts, pts_cap, counter, abort_flag);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(la_batch_kernel<16>), dim3(grid), dim3(64), 0, st, tasks, order, n, pk, sp, trbuf, outs, pools, pool_cap, pts, pts_cap, counter, abort_flag);
         break;
     default:
         return -1;

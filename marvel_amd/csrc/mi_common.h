@@ -23,6 +23,15 @@ struct DevSpecHost {
     DevSpec spec;   // table/score point at device memory
 };
 
+// Align_Spec of this library: the alignment constants plus the per-thread
+// overlap buffers the filter appends to (dalign/align.c:52-62, :260-276).
+struct SpecObj {
+    HostSpec hs;
+    MiAlignParams prm;
+    int nthreads, symmetric, only_identity, no_trace;
+    Overlap_IO_Buffer* iobuf;   // [nthreads]
+};
+
 void spec_build(HostSpec* s, double ave_corr, int tspace, const float* freq);
 // Device tables for these parameters (cached per parameter set; lives until exit).
 DevSpecHost* mi_spec_device(const MiAlignParams* prm);

@@ -1,0 +1,220 @@
+// Overlap buffers of an Align_Spec and the .las writer (host side).
+// Behaviour of dalign/align.c:5630-6056: records are buffered per thread,
+// then Write_Overlap_Buffer sorts all of them stably by (aread, bread,
+// COMP, abpos) and writes one .las file, or two split at lastRead, into
+// the run's RRR_BBBBB/ directories.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/marvl_align.h"
+#include "mi_common.h"
+
+using mi::SpecObj;
+
+namespace {
+
+constexpr int kBufferOvls = 500000;   // initial records over all threads (dalign/align.c:5651)
+
+std::string run_dir(int run_id, int block)
+{
+    char b[64];
+    snprintf(b, sizeof b, "%03d_%05d", run_id, block);
+    return b;
+}
+
+// "ROOT.<id>" -> (ROOT, id); no dot: (name, 0)
+int split_block_name(const char* name, std::string& root)
+{
+    const char* dot = strrchr(name, '.');
+    if (dot == nullptr) {
+        root = name;
+        return 0;
+    }
+    root.assign(name, dot - name);
+    return atoi(dot + 1);
+}
+
+bool write_las(const std::string& path, const std::vector<const Overlap*>& recs, size_t lo, size_t hi, int tspace,
+               int tbytes)
+{
+    FILE* f = fopen(path.c_str(), "w");
+    if (f == nullptr) {
+        fprintf(stderr, "[ERROR] - Write_Overlap_Buffer: Cannot open file %s for writing\n", path.c_str());
+        exit(1);
+    }
+    // one buffered write of the whole file
+    size_t bytes = 12;
+    for (size_t j = lo; j < hi; j++)
+        bytes += 40 + (recs[j]->path.trace ? (size_t)recs[j]->path.tlen * tbytes : 0);
+    std::vector<char> out(bytes);
+    const int64_t n = (int64_t)(hi - lo);
+    memcpy(out.data(), &n, 8);
+    memcpy(out.data() + 8, &tspace, 4);
+    size_t o = 12;
+    for (size_t j = lo; j < hi; j++) {
+        const Overlap* v = recs[j];
+        const int32_t rec[10] = {v->path.tlen,  v->path.diffs,      v->path.abpos, v->path.bbpos, v->path.aepos,
+                                 v->path.bepos, (int32_t)v->flags, v->aread,      v->bread,      0};
+        memcpy(out.data() + o, rec, 40);
+        o += 40;
+        if (v->path.trace) {
+            const size_t tb = (size_t)v->path.tlen * tbytes;
+            memcpy(out.data() + o, v->path.trace, tb);
+            o += tb;
+        }
+    }
+    const bool ok = fwrite(out.data(), 1, bytes, f) == bytes;
+    fclose(f);
+    return ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+Overlap_IO_Buffer* CreateOverlapBuffer(int nthreads, int tbytes, int no_trace)
+{
+    Overlap_IO_Buffer* b = (Overlap_IO_Buffer*)calloc(1, sizeof(Overlap_IO_Buffer));
+    if (b == nullptr) return nullptr;
+    b->omax = kBufferOvls / (nthreads > 0 ? nthreads : 1) + 1;
+    b->ovls = (Overlap*)malloc(sizeof(Overlap) * b->omax);
+    b->no_trace = no_trace;
+    if (b->ovls == nullptr) {
+        free(b);
+        return nullptr;
+    }
+    if (no_trace) return b;
+    if (tbytes < 1 || tbytes > 2) {
+        fprintf(stderr, "[ERROR] - Unsupported size of trace: %d!\n", tbytes);
+        free(b->ovls);
+        free(b);
+        return nullptr;
+    }
+    b->tbytes = tbytes;
+    b->tmax = (uint64_t)b->omax * 120 * tbytes;   // bytes
+    b->trace = malloc(b->tmax);
+    if (b->trace == nullptr) {
+        free(b->ovls);
+        free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+// Appends a record; its trace (tlen values of tbytes bytes) is copied.
+// Trace pointers of buffered records are kept valid across growth.
+int AddOverlapToBuffer(Overlap_IO_Buffer* b, Overlap* ovl, int tbytes)
+{
+    if (b == nullptr) return 1;
+    if (b->otop == b->omax) {
+        const int nmax = (int)(b->omax * 1.2) + 1000;
+        Overlap* o = (Overlap*)realloc(b->ovls, sizeof(Overlap) * nmax);
+        if (o == nullptr) return 1;
+        b->ovls = o;
+        b->omax = nmax;
+    }
+    const bool with_trace = ovl->path.trace != nullptr && b->no_trace == 0;
+    const uint64_t tb = with_trace ? (uint64_t)tbytes * ovl->path.tlen : 0;
+    if (with_trace && b->ttop + tb > b->tmax) {
+        const uint64_t nmax = (uint64_t)((b->ttop + tb) * 1.2) + 1000;
+        char* t = (char*)realloc(b->trace, nmax);
+        if (t == nullptr) return 1;
+        if (t != b->trace) {
+            uint64_t off = 0;
+            for (int i = 0; i < b->otop; i++)
+                if (b->ovls[i].path.trace != nullptr) {
+                    b->ovls[i].path.trace = t + off;
+                    off += (uint64_t)b->ovls[i].path.tlen * tbytes;
+                }
+        }
+        b->trace = t;
+        b->tmax = nmax;
+    }
+    Overlap* o = b->ovls + b->otop;
+    o->aread = ovl->aread;
+    o->bread = ovl->bread;
+    o->flags = ovl->flags;
+    o->path.abpos = ovl->path.abpos;
+    o->path.aepos = ovl->path.aepos;
+    o->path.bbpos = ovl->path.bbpos;
+    o->path.bepos = ovl->path.bepos;
+    o->path.diffs = ovl->path.diffs;
+    if (!with_trace) {
+        o->path.trace = nullptr;
+        o->path.tlen = 0;
+    } else {
+        o->path.tlen = ovl->path.tlen;
+        memcpy((char*)b->trace + b->ttop, ovl->path.trace, tb);
+        o->path.trace = (char*)b->trace + b->ttop;
+    }
+    b->otop++;
+    b->ttop += tb;
+    return 0;
+}
+
+Overlap_IO_Buffer* OVL_IO_Buffer(Align_Spec* espec) { return ((SpecObj*)espec)->iobuf; }
+int Num_Threads(Align_Spec* espec) { return ((SpecObj*)espec)->nthreads; }
+int Only_Identity(Align_Spec* espec) { return ((SpecObj*)espec)->only_identity; }
+int Symmetric(Align_Spec* espec) { return ((SpecObj*)espec)->symmetric; }
+
+void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblock, int lastRead)
+{
+    SpecObj* s = (SpecObj*)spec;
+    std::vector<const Overlap*> all;
+    size_t n = 0;
+    for (int i = 0; i < s->nthreads; i++) n += s->iobuf[i].otop;
+    all.reserve(n);
+    // thread buffers in order; the stable sort keeps that order for equal keys
+    for (int i = 0; i < s->nthreads; i++)
+        for (int j = 0; j < s->iobuf[i].otop; j++) {
+            const Overlap* o = s->iobuf[i].ovls + j;
+            if (!s->only_identity || o->aread == o->bread) all.push_back(o);
+        }
+    std::stable_sort(all.begin(), all.end(), [](const Overlap* l, const Overlap* r) {
+        if (l->aread != r->aread) return l->aread < r->aread;
+        if (l->bread != r->bread) return l->bread < r->bread;
+        if (COMP(l->flags) != COMP(r->flags)) return COMP(l->flags) < COMP(r->flags);
+        return l->path.abpos < r->path.abpos;
+    });
+    std::string aroot, broot;
+    const int aid = split_block_name(ablock, aroot);
+    const int bid = split_block_name(bblock, broot);
+    const int tspace = s->hs.tspace;
+    const int tbytes = s->iobuf[0].tbytes;
+    auto name = [&](const std::string& r1, int i1, const std::string& r2, int i2) {
+        std::string p;
+        if (i1 > 0) p = run_dir(RUN_ID, i1) + "/" + r1 + "." + std::to_string(i1) + "." + r2;
+        else p = r1 + "." + r2;
+        if (i2 > 0) p += "." + std::to_string(i2);
+        return p + ".las";
+    };
+    bool ok;
+    if (strcmp(ablock, bblock) == 0 || s->symmetric == 0) {
+        ok = write_las(name(aroot, aid, broot, bid), all, 0, all.size(), tspace, tbytes);
+    } else {
+        std::string p1 = name(aroot, aid, broot, bid), p2 = name(broot, bid, aroot, aid);
+        if (bid < aid) std::swap(p1, p2);
+        size_t j = 0;
+        while (j < all.size() && all[j]->aread <= lastRead) j++;
+        ok = write_las(p1, all, 0, j, tspace, tbytes) && write_las(p2, all, j, all.size(), tspace, tbytes);
+    }
+    if (!ok) {
+        fprintf(stderr, "[ERROR] - Write_Overlap_Buffer: write failed\n");
+        exit(1);
+    }
+}
+
+void Reset_Overlap_Buffer(Align_Spec* spec)
+{
+    SpecObj* s = (SpecObj*)spec;
+    for (int i = 0; i < s->nthreads; i++) {
+        s->iobuf[i].otop = 0;
+        s->iobuf[i].ttop = 0;
+    }
+}
+
+}  // extern "C"

@@ -24,7 +24,7 @@ constexpr int kTile = 4096;   // records per tile (64 lanes x 64 rounds)
 
 template <class Rec, class KeyOf>
 __global__ __launch_bounds__(256) void rs_hist(const Rec* __restrict__ in, long long n, int shift, KeyOf key,
-                                               unsigned* __restrict__ counts, int ntiles)
+                                               unsigned long long* __restrict__ counts, int ntiles)
 {
     __shared__ unsigned h[256];
     const int tid = threadIdx.x;
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void rs_hist(const Rec* __restrict__ in, long 
         if (g < n) atomicAdd(&h[(key(in[g]) >> shift) & 255], 1u);
     }
     __syncthreads();
-    counts[(size_t)tid * ntiles + blockIdx.x] = h[tid];
+    counts[(size_t)tid * ntiles + blockIdx.x] = h[tid];   // digit-major: scanning gives each (digit, tile) its base
 }
 
 template <class Rec, class KeyOf>

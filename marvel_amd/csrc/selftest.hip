@@ -85,13 +85,13 @@ extern "C" int mi_selftest(int which, const void* in, long long nin, const void*
     if (hipMalloc(&dctr, sizeof(int)) != hipSuccess || hipMemset(dctr, 0, sizeof(int)) != hipSuccess) goto done;
     switch (which) {
     case 0:
-        st_persistent<<<arg, 64>>>((int)(nout / 4), dctr, (int*)dout, abort_flag);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(st_persistent), dim3(arg), dim3(64), 0, 0, (int)(nout / 4), dctr, (int*)dout, abort_flag);
         break;
     case 1:
-        st_lanes<<<(int)(nin / 256), 64>>>((const int*)din, arg, (int*)dout);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(st_lanes), dim3((int)(nin / 256)), dim3(64), 0, 0, (const int*)din, arg, (int*)dout);
         break;
     case 2:
-        st_match<<<(nq + 63) / 64, 64>>>((const uint64_t*)din + 1, (const uint64_t*)din2 + 1, (const int*)dq, nq,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(st_match), dim3((nq + 63) / 64), dim3(64), 0, 0, (const uint64_t*)din + 1, (const uint64_t*)din2 + 1, (const int*)dq, nq,
                                          (int*)dout);
         break;
     default:

@@ -1,0 +1,145 @@
+"""MARVEL DB files for synthetic reads, and .las reading.
+
+write_db() produces the three files daligner reads for a DB ROOT
+(db/DB.h:164-249), as FA2db + DBsplit would for a FASTA of the same reads
+in the same order with one read per PacBio well:
+
+  ROOT.db     text stub: "files = %9d", one "  %9d %s %s" line per FASTA file,
+              then the DBsplit part "blocks = %9d", "size = %9lld" and
+              blocks+1 lines " %9d" with the first read of every block
+              (db/DBsplit.c:200-243: a block closes once its bases reach
+              size * 10^6)
+  .ROOT.idx   88-byte DB record (ureads, freq[4], maxlen, totlen; the rest
+              zero here where FA2db writes in-memory pointers) followed by one
+              32-byte record per read (rlen, boff into .bps, coff = -1,
+              flags = DB_BEST), db/FA2db.c:573-586
+  .ROOT.bps   per read ceil(rlen/4) bytes, base i at bits 6-2(i%4)
+              (Compress_Read, db/DB.c:261-281)
+freq[c] = (float)(count[c] / totlen) (db/FA2db.c:1101-1105).
+"""
+
+from __future__ import annotations
+
+import os
+import struct
+
+import numpy as np
+
+DB_BEST = 0x0800
+
+
+def pack_bps(read: np.ndarray) -> bytes:
+    n = read.shape[0]
+    m = (n + 3) // 4
+    pad = np.zeros(m * 4, dtype=np.uint8)
+    pad[:n] = read
+    q = pad.reshape(m, 4)
+    return ((q[:, 0] << 6) | (q[:, 1] << 4) | (q[:, 2] << 2) | q[:, 3]).astype(np.uint8).tobytes()
+
+
+def block_starts(lengths, size_mbp: int):
+    """First read of every block, DBsplit's rule (db/DBsplit.c:204-236)."""
+    size = size_mbp * 1000000
+    starts = [0]
+    tot = 0
+    ireads = 0
+    for i, L in enumerate(lengths):
+        ireads += 1
+        tot += int(L)
+        if tot >= size:
+            starts.append(i + 1)
+            tot = 0
+            ireads = 0
+    if ireads > 0:
+        starts.append(len(lengths))
+    return starts
+
+
+def write_db(root: str, reads, size_mbp: int = 400, prolog: str = "sim", fasta: str = "sim") -> dict:
+    """Writes ROOT.db, .ROOT.idx, .ROOT.bps; returns the DB statistics."""
+    d, name = os.path.split(root)
+    d = d or "."
+    lens = np.array([len(r) for r in reads], dtype=np.int64)
+    counts = np.zeros(4, dtype=np.int64)
+    for r in reads:
+        counts += np.bincount(r, minlength=4)[:4]
+    totlen = int(lens.sum())
+    maxlen = int(lens.max()) if len(reads) else 0
+    freq = [np.float32((1.0 * int(c)) / totlen) if totlen else np.float32(0) for c in counts]
+    with open(os.path.join(d, "." + name + ".bps"), "wb") as fb, open(os.path.join(d, "." + name + ".idx"), "wb") as fi:
+        hdr = struct.pack("<i4fiq", len(reads), *[float(f) for f in freq], maxlen, totlen)
+        fi.write(hdr + b"\0" * (88 - len(hdr)))
+        off = 0
+        recs = []
+        for r in reads:
+            b = pack_bps(r)
+            fb.write(b)
+            recs.append(struct.pack("<iiqqii", len(r), 0, off, -1, DB_BEST, 0))
+            off += len(b)
+        fi.write(b"".join(recs))
+    starts = block_starts(lens, size_mbp)
+    with open(os.path.join(d, name + ".db"), "w") as fs:
+        fs.write("files = %9d\n" % 1)
+        fs.write("  %9d %s %s\n" % (len(reads), prolog, fasta))
+        fs.write("blocks = %9d\n" % (len(starts) - 1))
+        fs.write("size = %9d\n" % size_mbp)
+        for s in starts:
+            fs.write(" %9d\n" % s)
+    return {"nreads": len(reads), "totlen": totlen, "maxlen": maxlen, "freq": [float(f) for f in freq],
+            "blocks": len(starts) - 1}
+
+
+def read_idx(root: str):
+    """(header fields, per-read (rlen, boff) array) of a DB's .idx."""
+    d, name = os.path.split(root)
+    with open(os.path.join(d or ".", "." + name + ".idx"), "rb") as f:
+        raw = f.read()
+    ureads, f0, f1, f2, f3, maxlen, totlen = struct.unpack_from("<i4fiq", raw, 0)
+    rec = np.frombuffer(raw, dtype=np.dtype([("rlen", "<i4"), ("pad", "<i4"), ("boff", "<i8"), ("coff", "<i8"),
+                                             ("flags", "<i4"), ("pad2", "<i4")]), count=ureads, offset=88)
+    return {"ureads": ureads, "freq": [f0, f1, f2, f3], "maxlen": maxlen, "totlen": totlen}, rec
+
+
+# ---- .las (dalign/align.c:3325-3350) -----------------------------------------
+
+LAS_REC = np.dtype([("tlen", "<i4"), ("diffs", "<i4"), ("abpos", "<i4"), ("bbpos", "<i4"), ("aepos", "<i4"),
+                    ("bepos", "<i4"), ("flags", "<u4"), ("aread", "<i4"), ("bread", "<i4"), ("pad", "<i4")])
+
+
+def read_las(path: str):
+    """(tspace, records structured array, list of trace byte strings)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    n, tspace = struct.unpack_from("<qi", raw, 0)
+    tbytes = 1 if tspace <= 125 else 2
+    o = 12
+    recs = np.empty(n, dtype=LAS_REC)
+    traces = []
+    for i in range(n):
+        r = np.frombuffer(raw, dtype=LAS_REC, count=1, offset=o)[0]
+        recs[i] = r
+        o += 40
+        tb = int(r["tlen"]) * tbytes
+        traces.append(raw[o:o + tb])
+        o += tb
+    if o != len(raw):
+        raise ValueError(f"{path}: {len(raw) - o} trailing bytes")
+    return tspace, recs, traces
+
+
+def first_difference(path_a: str, path_b: str):
+    """None when the files are byte-identical, else a description of the first differing record."""
+    with open(path_a, "rb") as f:
+        a = f.read()
+    with open(path_b, "rb") as f:
+        b = f.read()
+    if a == b:
+        return None
+    ta, ra, xa = read_las(path_a)
+    tb, rb, xb = read_las(path_b)
+    if ta != tb:
+        return f"trace spacing {ta} != {tb}"
+    for i in range(min(len(ra), len(rb))):
+        if ra[i].tobytes() != rb[i].tobytes() or xa[i] != xb[i]:
+            return f"record {i}: {ra[i]} vs {rb[i]}" + ("" if xa[i] == xb[i] else " (traces differ)")
+    return f"record counts {len(ra)} != {len(rb)}"

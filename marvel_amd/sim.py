@@ -38,6 +38,31 @@ def random_genome(length: int, seed: int) -> np.ndarray:
     return rng.integers(0, 4, size=length, dtype=np.uint8)
 
 
+def repeat_genome(length: int, seed: int, families=(), tandems=()) -> np.ndarray:
+    """iid genome with interspersed repeat families and tandem repeats written over it.
+
+    families: (consensus length, copies, divergence) -- every copy is the
+              consensus through mutate() with the divergence split 9:4:2 into
+              insertion / deletion / substitution rates, at a uniform position;
+    tandems:  (period, length) -- a random unit of `period` bases repeated to
+              `length` bases at a uniform position.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = rng.integers(0, 4, size=length, dtype=np.uint8)
+    for cl, copies, div in families:
+        cons = rng.integers(0, 4, size=cl, dtype=np.uint8)
+        for _ in range(copies):
+            c = mutate(cons, rng, div * 9 / 15, div * 4 / 15, div * 2 / 15)
+            at = int(rng.integers(0, length - c.shape[0]))
+            g[at:at + c.shape[0]] = c
+    for period, tl in tandems:
+        unit = rng.integers(0, 4, size=period, dtype=np.uint8)
+        t = np.tile(unit, tl // period + 1)[:tl]
+        at = int(rng.integers(0, length - tl))
+        g[at:at + tl] = t
+    return g
+
+
 def lognormal_params(mean: float, sd: float):
     s2 = math.log(1.0 + (sd * sd) / (mean * mean))
     return math.log(mean) - s2 / 2.0, math.sqrt(s2)
@@ -107,6 +132,14 @@ def write_fasta(path: str, reads, prefix: str = "sim", width: int = 80) -> None:
 def config_small(seed: int = 1):
     """200 kbp iid genome, 20x, lognormal(4 kb, 2 kb, min 500), 9/4/2 % errors."""
     g = random_genome(200_000, seed)
+    return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
+
+
+def config_small_repeats(seed: int = 11):
+    """config_small's read model over a 200 kbp genome with repeat families
+    (3 kb x 6 at 5 %, 1.5 kb x 10 at 2 %) and tandem repeats (periods 2, 7, 30)."""
+    g = repeat_genome(200_000, seed, families=((3000, 6, 0.05), (1500, 10, 0.02)),
+                      tandems=((2, 3000), (7, 2000), (30, 4000)))
     return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
 
 

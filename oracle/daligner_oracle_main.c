@@ -65,7 +65,15 @@ int main(int argc, char **argv)
         case 'w': o.binshift = atoi(optarg); break;
         case 'h': o.hitmin = atoi(optarg); break;
         case 't': o.suppress = atoi(optarg); break;
-        case 'H': o.hgap_min = atoi(optarg); break;
+        case 'H':
+            /* the reference parses -H into a local of main that shadows the
+             * global HGAP_MIN the filter reads (dalign/daligner.c:641,
+             * :709-711), so the option has no effect on the output */
+            if (atoi(optarg) < 0) {
+                fprintf(stderr, "invalid HGAP threshold of %d\n", atoi(optarg));
+                return 1;
+            }
+            break;
         case 'e': o.ave_corr = atof(optarg); break;
         case 'l': l = atoi(optarg); break;
         case 's': o.tspace = atoi(optarg); break;

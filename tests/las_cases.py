@@ -1,0 +1,69 @@
+"""daligner workloads with a .las golden from the compiled reference.
+
+Each case names a read set (marvel_amd.sim, regenerated from its seed), the
+DBsplit block size and the daligner arguments.  The goldens
+(tests/golden/las/goldens.json) hold the SHA-256 of every .las the reference
+daligner wrote for the case; tests run the oracle restatement (CPU) and the
+GPU daligner on the same DB and compare.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+
+from marvel_amd import dbio, sim
+
+DATA = {
+    "small": lambda: sim.config_small(1),
+    "small_rep": lambda: sim.config_small_repeats(11),
+}
+
+# name -> (data, DBsplit size in Mbp, daligner flags, block arguments)
+CASES = {
+    "c1_j8": ("small", 400, ["-j8", "-M16"], ["C1.1", "C1.1"]),
+    "c1_j1": ("small", 400, ["-j1", "-M16"], ["C1.1", "C1.1"]),
+    "c1_j64": ("small", 400, ["-j64", "-M16"], ["C1.1", "C1.1"]),
+    "c1_asym": ("small", 400, ["-A", "-j4", "-M16"], ["C1.1", "C1.1"]),
+    "c1_s200": ("small", 400, ["-s200", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "c1_notrace": ("small", 400, ["-T", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "c1_k12": ("small", 400, ["-k12", "-w5", "-h30", "-l500", "-e.80", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "c1_hgap": ("small", 400, ["-H6000", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "c1_blocks_12": ("small", 2, ["-j8", "-M16"], ["C1.1", "C1.2"]),
+    "c1_blocks_21": ("small", 2, ["-j4", "-M16"], ["C1.2", "C1.1"]),
+    "c1_blocks_asym": ("small", 2, ["-A", "-j8", "-M16"], ["C1.2", "C1.1", "C1.2"]),
+    "rep_j8": ("small_rep", 400, ["-j8", "-M16"], ["C1.1", "C1.1"]),
+    "rep_t30": ("small_rep", 400, ["-t30", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "rep_j16_l500": ("small_rep", 400, ["-j16", "-l500", "-M16"], ["C1.1", "C1.1"]),
+}
+
+_reads = {}
+
+
+def reads_of(data: str):
+    if data not in _reads:
+        _reads[data] = DATA[data]().reads
+    return _reads[data]
+
+
+def make_db(case: str, d: str) -> None:
+    data, split, _, _ = CASES[case]
+    os.makedirs(d, exist_ok=True)
+    dbio.write_db(os.path.join(d, "C1"), reads_of(data), size_mbp=split)
+
+
+def run(exe: str, case: str, d: str, timeout: float = 600) -> dict:
+    """Runs `exe` (a daligner-compatible binary) in DB directory d; returns
+    {relative .las path: sha256}."""
+    _, _, flags, blocks = CASES[case]
+    r = subprocess.run([exe] + flags + blocks, cwd=d, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"{exe} {case}: rc {r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    out = {}
+    for dp, _, fs in os.walk(d):
+        for f in fs:
+            if f.endswith(".las"):
+                p = os.path.join(dp, f)
+                out[os.path.relpath(p, d)] = hashlib.sha256(open(p, "rb").read()).hexdigest()
+    return out

@@ -7,14 +7,8 @@
 #include <thread>
 #include <vector>
 
-#include "wave_emu.hpp"
-#define MI_CPU_EMU 1
+#include "hip_cpu.hpp"
 #include "../../marvel_amd/csrc/la_wave.hpp"
-
-namespace emu {
-Ctx* g = nullptr;
-thread_local int lane = 0;
-}  // namespace emu
 
 static int g_abort = 0;
 static const double kBias[10] = {.690, .690, .690, .690, .780, .850, .900, .933, .966, 1.000};

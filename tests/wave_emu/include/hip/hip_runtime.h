@@ -1,4 +1,4 @@
-// Stand-in for <hip/hip_runtime.h> when the device code is compiled for the
-// CPU wave emulator (tests only): maps the intrinsics to wave_emu.hpp.
+// Stand-in for <hip/hip_runtime.h> when the library's HIP sources are
+// compiled for the CPU (tests only): see ../../hip_cpu.hpp.
 #pragma once
-#include "../../wave_emu.hpp"
+#include "../../hip_cpu.hpp"

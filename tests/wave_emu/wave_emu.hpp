@@ -5,7 +5,9 @@
 // exchange, so all 64 lanes must reach the same sequence of them (true for
 // the aligner: its cross-lane operations sit in wave-uniform control flow).
 #pragma once
-#include <barrier>
+#include <sched.h>
+
+#include <atomic>
 #include <climits>
 #include <cstdint>
 #include <cstring>
@@ -16,8 +18,25 @@
 #define __host__
 
 namespace emu {
+// Generation barrier that yields while waiting: the lanes outnumber the host
+// cores, and a yield hands the core to a lane that has not arrived yet.
+struct YieldBarrier {
+    explicit YieldBarrier(int parties) : n(parties) {}
+    int n;
+    std::atomic<int> count{0};
+    std::atomic<int> gen{0};
+    void arrive_and_wait()
+    {
+        const int g0 = gen.load(std::memory_order_acquire);
+        if (count.fetch_add(1, std::memory_order_acq_rel) == n - 1) {
+            count.store(0, std::memory_order_relaxed);
+            gen.fetch_add(1, std::memory_order_acq_rel);
+        } else
+            while (gen.load(std::memory_order_acquire) == g0) sched_yield();
+    }
+};
 struct Ctx {
-    std::barrier<> bar{64};
+    YieldBarrier bar{64};
     long long slot[64];
 };
 extern Ctx* g;

@@ -95,8 +95,12 @@ typedef struct {
     long long edge_fixups;          /* pairs re-run for the band cell shared across pairs */
     long long retries;              /* pairs re-run with wider windows / larger arenas */
     int limit;                      /* k-mer match cap from MEM_LIMIT */
-    double t_filter, t_report, t_copy;   /* seconds: index merge + seed sort, report kernel, copy-out */
+    double t_filter, t_report, t_copy;   /* seconds (host clock): merge + seed sort, report launches, copy-out */
     double t_total;
+    double t_report_kernel;         /* seconds in the report kernel (HIP events on the filter stream) */
+    long long report_launches;
+    double t_seed_sort;             /* seconds in the seed radix sort (HIP events) */
+    long long seed_sort_passes;     /* 8-bit digit passes of that sort */
 } MiFilterStats;
 
 /* Match_Filter returning an error code (MI_ERR_*) instead of exiting. */
@@ -110,6 +114,16 @@ void mi_filter_stats(MiFilterStats *out);
  * Sort_Kmers).  `cblock` describes the complemented block (read lengths,
  * ufirst, maxlen); its bases are not read. */
 void *mi_sort_kmers_complement(void *table, HITS_DB *cblock, int *len);
+
+/* Sort_Kmers in two steps, for callers that keep a block resident in HBM:
+ * mi_block_device uploads and 2-bit packs the block's reads (no k-mers);
+ * mi_sort_kmers_device builds a k-mer index of such a block without
+ * touching host memory.  Both handles are released by mi_free_kmer_index;
+ * only the second is a table for Match_Filter. */
+void *mi_block_device(HITS_DB *block);
+void *mi_sort_kmers_device(void *dblock, int *len);
+/* Waits for all work of this process on the GPU. */
+int mi_device_synchronize(void);
 void mi_free_kmer_index(void *table);
 
 /* Block loading for hosts without the reference's DB library: the

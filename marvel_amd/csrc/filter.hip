@@ -285,7 +285,7 @@ static void upload_layout(KIdx* x)
     }
 }
 
-static KIdx* index_from_block(HITS_DB* blk)
+static KIdx* index_from_block(HITS_DB* blk, bool keys = true)
 {
     std::unique_ptr<KIdx> x(new KIdx);
     layout_reads(x.get(), blk);
@@ -310,6 +310,29 @@ static KIdx* index_from_block(HITS_DB* blk)
         HC(hipGetLastError());
         HC(hipStreamSynchronize(st));
     }
+    if (keys) build_keys(x.get());
+    return x.release();
+}
+
+// k-mer index of reads already packed in HBM (the handle of mi_block_device)
+static KIdx* index_from_device(const KIdx* src)
+{
+    std::unique_ptr<KIdx> x(new KIdx);
+    x->nreads = src->nreads;
+    x->maxlen = src->maxlen;
+    x->ufirst = src->ufirst;
+    x->part = src->part;
+    x->fmt = src->fmt;
+    x->rlen = src->rlen;
+    x->w0 = src->w0;
+    x->nwords = src->nwords;
+    hipStream_t st = fstream();
+    x->pk.ensure(x->nwords);
+    x->dw0.ensure(x->nreads + 1);
+    x->dlen.ensure(x->nreads + 1);
+    HC(hipMemcpyAsync(x->pk.p, src->pk.p, sizeof(uint64_t) * x->nwords, hipMemcpyDeviceToDevice, st));
+    HC(hipMemcpyAsync(x->dw0.p, src->dw0.p, sizeof(long long) * (x->nreads + 1), hipMemcpyDeviceToDevice, st));
+    HC(hipMemcpyAsync(x->dlen.p, src->dlen.p, sizeof(int) * (x->nreads + 1), hipMemcpyDeviceToDevice, st));
     build_keys(x.get());
     return x.release();
 }
@@ -364,6 +387,28 @@ static KIdx* as_index(void* t)
 
 static MiFilterStats g_stats;
 
+// HIP events on the filter stream around the kernels whose rates are reported
+struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    void record_start()
+    {
+        if (a == nullptr) {
+            HC(hipEventCreate(&a));
+            HC(hipEventCreate(&b));
+        }
+        HC(hipEventRecord(a, fstream()));
+    }
+    void record_stop() { HC(hipEventRecord(b, fstream())); }
+    double seconds()
+    {
+        HC(hipEventSynchronize(b));
+        float ms = 0;
+        HC(hipEventElapsedTime(&ms, a, b));
+        return ms * 1e-3;
+    }
+};
+static EvPair g_ev_report, g_ev_sort;
+
 struct Level {
     int nch, pool_cap, mcap;
     long long mtcap;
@@ -391,6 +436,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     S.nkmers_a = A->nk;
     S.nkmers_b = B->nk;
     if (A->nk == 0 || B->nk == 0) return;
+    if (A->K == 0 || B->K == 0) throw Fail("device block without a k-mer index (use mi_sort_kmers_device)", MI_ERR_ARG);
     if (A->K != K || B->K != K) throw Fail("index built with another k-mer length", MI_ERR_ARG);
     ensure_groups(A);
     const long long na = A->nk, nb = B->nk, ng = A->ng;
@@ -410,7 +456,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, be.p, B->keys.p, B->fmt, self ? 1 : 0,
                                        cnt.p);
     g_scan.run(cnt.p, off.p, na);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_totals), dim3(nblk(ng)), dim3(256), 0, st, A->gstart.p, ng, off.p, gtot.p, hist.p, MAXGRAM);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_totals), dim3(nblk(ng, 256 * 16)), dim3(256), 0, st, A->gstart.p, ng, off.p, gtot.p, hist.p, MAXGRAM);
     HC(hipGetLastError());
     std::vector<u64> h(MAXGRAM);
     HC(hipMemcpyAsync(h.data(), hist.p, sizeof(u64) * MAXGRAM, hipMemcpyDeviceToHost, st));
@@ -454,7 +500,11 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     HC(hipGetLastError());
     const int pa = A->fmt.pbits, ra = A->fmt.rbits, rb = B->fmt.rbits;
     if (pa + ra + rb > 64) throw Fail("blocks too large for 64-bit seed keys", MI_ERR_ARG);
+    g_ev_sort.record_start();
     Seed* hits = radix_sort(s1.p, s2.p, nh, 0, pa + ra + rb, KeySeed{pa, pa + ra});
+    g_ev_sort.record_stop();
+    S.t_seed_sort += g_ev_sort.seconds();
+    S.seed_sort_passes += (pa + ra + rb + 7) / 8;
     Seed* spare = (hits == s1.p) ? s2.p : s1.p;
     (void)spare;
 
@@ -511,7 +561,9 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     const int w = maxdiag - mindiag + 1;
     const int maxtp_all = 2 * (std::max(A->maxlen, B->maxlen) / TS + 2);
     const int cus = mi_num_cus();
-    const Level levels[4] = {{1, 1 << 17, 256, 1 << 18, cus * 8},
+    // waves per level: up to 8 per SIMD for the 64-window kernel; the per-wave
+    // pools are sized for typical pairs, outliers go to the larger levels
+    const Level levels[4] = {{1, 1 << 15, 128, 1 << 16, cus * 32},
                              {4, 1 << 17, 256, 1 << 18, cus * 8},
                              {16, 1 << 17, 256, 1 << 18, cus * 4},
                              {16, 1 << 20, 4096, 1LL << 22, std::min(cus, 256)}};
@@ -631,15 +683,19 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         b.ocap = ocap;
         b.otr = otr.p;
         b.otcap = otcap;
+        g_ev_report.record_start();
         switch (L.nch) {
         case 1: launch_report<1>(b, grid); break;
         case 4: launch_report<4>(b, grid); break;
         default: launch_report<16>(b, grid); break;
         }
         HC(hipGetLastError());
+        g_ev_report.record_stop();
         const int wd = mi_wait_watchdog(st, "Match_Filter report kernel");
         if (wd > 0) throw Fail("report kernel exceeded the watchdog deadline and was stopped", MI_ERR_TIMEOUT);
         if (wd < 0) throw Fail("report kernel failed on the device");
+        S.t_report_kernel += g_ev_report.seconds();
+        S.report_launches += 1;
         u64 oc2[2];
         HC(hipMemcpy(oc2, ocount.p, sizeof oc2, hipMemcpyDeviceToHost));
         HC(hipMemcpy(hstatus.data(), status.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
@@ -848,6 +904,30 @@ void* Sort_Kmers(HITS_DB* block, int* len)
     }
     return r;
 }
+
+void* mi_block_device(HITS_DB* block)
+{
+    try {
+        return index_from_block(block, false);
+    } catch (const Fail& e) {
+        mi_set_error(e.what());
+        return nullptr;
+    }
+}
+
+void* mi_sort_kmers_device(void* dblock, int* len)
+{
+    try {
+        KIdx* x = index_from_device(as_index(dblock));
+        if (len) *len = (int)std::min<long long>(x->nk, INT32_MAX);
+        return x;
+    } catch (const Fail& e) {
+        mi_set_error(e.what());
+        return nullptr;
+    }
+}
+
+int mi_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : MI_ERR_DEVICE; }
 
 void* mi_sort_kmers_complement(void* table, HITS_DB* cblock, int* len)
 {

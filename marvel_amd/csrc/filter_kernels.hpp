@@ -203,16 +203,33 @@ __global__ void k_counts(const uint64_t* __restrict__ ak, KeyFmt fa, long long n
     cnt[i] = (unsigned long long)(e - s);
 }
 
-// group totals (from the exclusive scan of counts) and their histogram (hitgram)
-__global__ void k_group_totals(const long long* __restrict__ gstart, long long ng,
-                               const unsigned long long* __restrict__ cscan, unsigned long long* __restrict__ gtot,
-                               unsigned long long* __restrict__ hist, int maxgram)
+// group totals (from the exclusive scan of counts) and their histogram
+// (hitgram).  Most groups have tiny totals, so the low bins are counted in
+// LDS per workgroup and flushed once; only totals >= kLdsBins go straight
+// to the global histogram.
+constexpr int kLdsBins = 1024;
+
+__global__ __launch_bounds__(256) void k_group_totals(const long long* __restrict__ gstart, long long ng,
+                                                      const unsigned long long* __restrict__ cscan,
+                                                      unsigned long long* __restrict__ gtot,
+                                                      unsigned long long* __restrict__ hist, int maxgram)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    const unsigned long long t = cscan[gstart[g + 1]] - cscan[gstart[g]];
-    gtot[g] = t;
-    if (t < (unsigned long long)maxgram) atomicAdd(&hist[t], 1ull);
+    __shared__ unsigned h[kLdsBins];
+    for (int i = threadIdx.x; i < kLdsBins; i += 256) h[i] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * 256 * 16;
+    for (int k = 0; k < 16; k++) {
+        const long long g = base + k * 256 + threadIdx.x;
+        if (g < ng) {
+            const unsigned long long t = cscan[gstart[g + 1]] - cscan[gstart[g]];
+            gtot[g] = t;
+            if (t < (unsigned long long)kLdsBins && t < (unsigned long long)maxgram) atomicAdd(&h[t], 1u);
+            else if (t < (unsigned long long)maxgram) atomicAdd(&hist[t], 1ull);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLdsBins && i < maxgram; i += 256)
+        if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
 }
 
 // groups whose total reaches the cap are dropped (merge_thread, dalign/filter.c:1148-1154)

@@ -18,6 +18,7 @@ from marvel_amd import dbio, sim
 DATA = {
     "small": lambda: sim.config_small(1),
     "small_rep": lambda: sim.config_small_repeats(11),
+    "5mbp": lambda: sim.config_5mbp(2),
 }
 
 # name -> (data, DBsplit size in Mbp, daligner flags, block arguments)
@@ -36,7 +37,10 @@ CASES = {
     "rep_j8": ("small_rep", 400, ["-j8", "-M16"], ["C1.1", "C1.1"]),
     "rep_t30": ("small_rep", 400, ["-t30", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "rep_j16_l500": ("small_rep", 400, ["-j16", "-l500", "-M16"], ["C1.1", "C1.1"]),
+    # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)
+    "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
 }
+LARGE = {"c2_j8"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
 
 _reads = {}
 
@@ -48,9 +52,9 @@ def reads_of(data: str):
 
 
 def make_db(case: str, d: str) -> None:
-    data, split, _, _ = CASES[case]
+    data, split, _, blocks = CASES[case]
     os.makedirs(d, exist_ok=True)
-    dbio.write_db(os.path.join(d, "C1"), reads_of(data), size_mbp=split)
+    dbio.write_db(os.path.join(d, blocks[0].split(".")[0]), reads_of(data), size_mbp=split)
 
 
 def run(exe: str, case: str, d: str, timeout: float = 600) -> dict:

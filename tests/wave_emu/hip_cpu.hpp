@@ -150,6 +150,19 @@ inline hipError_t hipGetDeviceCount(int* n)
     *n = 1;
     return hipSuccess;
 }
+typedef struct EmuEvent* hipEvent_t;
+inline hipError_t hipEventCreate(hipEvent_t* e)
+{
+    *e = (hipEvent_t)0x1;
+    return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t)
+{
+    *ms = 0;
+    return hipSuccess;
+}
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
 {
     const char* e = getenv("MI_EMU_CUS");

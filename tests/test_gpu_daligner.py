@@ -51,3 +51,51 @@ def test_gpu_daligner_matches_reference(case):
                 diffs = {rel: dbio.first_difference(os.path.join(d, rel), os.path.join(d2, rel))
                          for rel in GOLD[case]["las"] if os.path.exists(os.path.join(d, rel))}
             pytest.fail(f"{case}: {got} != {GOLD[case]['las']}; first differences vs oracle: {diffs}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["c1_j8", "rep_j8", "c2_j8"])
+def test_gpu_daligner_in_process(case, tmp_path):
+    """Same comparison through the library loaded in this process (Python
+    mirror of Set_Filter_Params / Sort_Kmers / Match_Filter /
+    Write_Overlap_Buffer), with the block indexed from HBM-resident reads."""
+    from marvel_amd.daligner import Daligner
+
+    _, _, flags, blocks = las_cases.CASES[case]
+    assert blocks[0] == blocks[1], "self comparisons only"
+    opts = {f[1]: f[2:] for f in flags}
+    las_cases.make_db(case, str(tmp_path))
+    dal = Daligner()
+    nthreads = int(opts.get("j", "4"))
+    dal.configure(nthreads=nthreads, mem_limit_gb=int(opts["M"]), minover=2 * int(opts.get("l", "1000")))
+    dal.set_filter_params(int(opts.get("k", "14")), int(opts.get("w", "6")), int(opts.get("t", "0")),
+                          int(opts.get("h", "35")))
+    A = dal.open_block(str(tmp_path / blocks[0]))
+    spec = dal.new_align_spec(float(opts.get("e", ".70")), int(opts.get("s", "100")), list(A.freq), nthreads=nthreads)
+    dev = dal.block_device(A)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        dal.compare_self(A, spec, 1, blocks[0], dev=dev)
+    finally:
+        os.chdir(cwd)
+        dal.free_index(dev)
+    assert _shas(str(tmp_path)) == GOLD[case]["las"]
+
+
+INTEG = os.path.join(ROOT, "oracle", "_ref", "daligner_on_libmarvl")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(INTEG), reason="integration binary not built (make -C oracle integ)")
+@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "c2_j8"])
+def test_reference_driver_on_library(case):
+    """The reference's own daligner.c and DB code, linked against
+    libmarvl_align instead of its filter.c / align.c, writes the golden .las."""
+    print(f"[reference driver on libmarvl_align] case {case}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        las_cases.make_db(case, d)
+        _, _, flags, blocks = las_cases.CASES[case]
+        r = subprocess.run([INTEG] + flags + blocks, cwd=d, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert _shas(d) == GOLD[case]["las"]

@@ -118,6 +118,20 @@ void or_ovlset_sort(or_ovlset *s);
 int or_write_las(const char *path, const or_ovlset *s, int64_t lo, int64_t hi, int tspace);
 void or_ovlset_free(or_ovlset *s);
 
+/* Compute_Trace_PTS (dalign/align.c:5286-5397, iter_np :4556-4916) on the
+ * A-view path with its uint16 trace points; mode -1 LOWERMOST, 0 GREEDIEST,
+ * 1 UPPERMOST.  Writes the int trace (indel positions) to out.  Returns 0,
+ * -1 for a self comparison on diagonal 0, -3 when cap is too small. */
+int or_trace_pts(const char *aseq, const char *bseq, const or_path *path, const uint16_t *points,
+                 int trace_spacing, int mode, int *out, int64_t cap, int *ntrace, int *diffs);
+
+/* Task driver in the layout of oracle/trace_vectors.c: per task
+ * {status, a_tlen, 3 x {diffs, ntrace, trace[ntrace]}} appended to res
+ * (int32 words).  Returns the number of words written, or -1 when rcap is
+ * too small. */
+int64_t or_trace_tasks(double ave_corr, int tspace, const float freq[4], const char *seq, int ntasks,
+                       const int64_t *aoff, const int64_t *boff, const int32_t *par, int32_t *res, int64_t rcap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,3 +59,54 @@ def run_tasks(ts):
         bt = tr[o + f[5]:o + f[5] + f[11]].copy()
         out.append((f, at, bt))
     return out, res[:, 12].copy()
+
+
+class OrPath(C.Structure):
+    _fields_ = [("tlen", C.c_int), ("diffs", C.c_int), ("abpos", C.c_int), ("bbpos", C.c_int),
+                ("aepos", C.c_int), ("bepos", C.c_int)]
+
+
+def _trace_handle():
+    h = handle()
+    if not getattr(h, "_trace_ready", False):
+        h.or_trace_tasks.restype = C.c_int64
+        h.or_trace_tasks.argtypes = [C.c_double, C.c_int, C.POINTER(C.c_float), C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        h.or_trace_pts.restype = C.c_int
+        h.or_trace_pts.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrPath), C.c_void_p, C.c_int, C.c_int,
+                                   C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        h._trace_ready = True
+    return h
+
+
+def run_trace_tasks(ts) -> np.ndarray:
+    """Oracle Local_Alignment + Compute_Trace_PTS (3 modes) over a TaskSet, in
+    the result layout of oracle/trace_vectors.c (int32 words)."""
+    h = _trace_handle()
+    seq, ao, bo = ts.blob()
+    par = np.ascontiguousarray(ts.par, dtype=np.int32)
+    cap = int(sum(2 + 3 * (2 + 2 * (int(p[0]) + int(p[1]))) for p in ts.par)) + 64
+    res = np.zeros(cap, dtype=np.int32)
+    freq = (C.c_float * 4)(*ts.freq)
+    n = h.or_trace_tasks(ts.ave_corr, ts.tspace, freq, seq.ctypes.data, ts.n, ao.ctypes.data, bo.ctypes.data,
+                         par.ctypes.data, res.ctypes.data, cap)
+    if n < 0:
+        raise RuntimeError("oracle trace capacity")
+    return res[:n].copy()
+
+
+def trace_segment(a: np.ndarray, b: np.ndarray, tspace: int, mode: int):
+    """Compute_Trace_PTS of a one-segment path a[0..M) x b[0..N) (M <= tspace)
+    with the given recorded difference count -> (diffs, trace)."""
+    h = _trace_handle()
+    buf = np.concatenate([[4], a, [4], b, [4]]).astype(np.uint8)
+    M, N = len(a), len(b)
+    path = OrPath(2, 0, 0, 0, M, N)
+    pts = np.array([M + N, N], dtype=np.uint16)
+    out = np.zeros(M + N + 16, dtype=np.int32)
+    nt, df = C.c_int(), C.c_int()
+    rc = h.or_trace_pts(buf.ctypes.data + 1, buf.ctypes.data + 2 + M, C.byref(path), pts.ctypes.data, tspace, mode,
+                        out.ctypes.data, len(out), C.byref(nt), C.byref(df))
+    if rc:
+        raise RuntimeError(f"or_trace_pts {rc}")
+    return df.value, out[:nt.value].copy()

@@ -77,6 +77,16 @@ void Free_Work_Data(Work_Data *work);
 Path *Local_Alignment(Alignment *align, Work_Data *work, Align_Spec *spec, int low, int hgh, int anti, int lbord,
                       int hbord);
 
+/* Compute_Trace_PTS: replaces dalign/align.c:5286-5397 (dalign/align.h:224).
+ * Runs on the GPU.  Same contract: align->path (A view, uint16 trace
+ * points) gets an int trace of indel positions that lives in `work` until
+ * the next call, tlen = its length, diffs = the summed segment distances.
+ * mode: LOWERMOST -1, GREEDIEST 0, UPPERMOST 1.  `debug` is ignored. */
+#define LOWERMOST -1
+#define GREEDIEST 0
+#define UPPERMOST 1
+int Compute_Trace_PTS(Alignment *align, Work_Data *work, int trace_spacing, int mode, int debug);
+
 /* Overlap record helpers: replace dalign/align.c:3328-3360, :3419-3449
  * (dalign/align.h:309-318); host code, byte-compatible .las records. */
 int Read_Overlap(FILE *input, Overlap *ovl);
@@ -152,6 +162,33 @@ typedef struct {
  * ntrace uint16 values) in task order, A then B. */
 int mi_la_batch_host(const MiAlignParams *prm, const uint64_t *packed, long long npacked, const MiLaTask *tasks,
                      int ntasks, MiLaResult *results, uint16_t *traces, long long ntrace);
+
+/* One Compute_Trace_PTS task: the A-view path of an alignment (its uint16
+ * trace points at `points` in the points array) between two 2-bit packed
+ * reads laid out as for MiLaTask (the B read already complemented when the
+ * alignment is COMP). */
+typedef struct {
+    long long aword, bword;
+    int alen, blen;
+    int abpos, bbpos, aepos, bepos;
+    int tlen;             /* number of uint16 trace values (pairs) */
+    long long points;     /* offset of the trace values in the points array */
+    int selfie;           /* A and B are the same sequence object */
+} MiTraceTask;
+
+typedef struct {
+    int status;           /* 0 ok; 2: a segment of a self comparison starts on diagonal 0 */
+    int diffs;            /* summed segment distances */
+    int ntrace;           /* ints in the trace */
+    long long trace;      /* offset of the trace in the output array */
+} MiTraceResult;
+
+/* Compute_Trace_PTS on a batch of paths (one wave per path, one lane per
+ * trace-point segment); mode LOWERMOST (-1), GREEDIEST (0) or UPPERMOST (1).
+ * Traces are appended to `trace` (capacity ntrace ints) in task order. */
+int mi_trace_pts_host(int trace_spacing, int mode, const uint64_t *packed, long long npacked, const uint16_t *points,
+                      long long npoints, const MiTraceTask *tasks, int ntasks, MiTraceResult *results, int *trace,
+                      long long ntrace);
 
 /* Alignment parameters New_Align_Spec derives (ave_path, mscore, dscore). */
 int mi_align_params(const MiAlignParams *prm, int *ave_path, int *mscore, int *dscore);

@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from .lib import MiLaResult, MiLaTask, lib, params
+from .lib import MiLaResult, MiLaTask, MiTraceResult, MiTraceTask, lib, params
 
 
 def la_batch(ts, return_counters: bool = False):
@@ -47,3 +47,43 @@ def la_batch(ts, return_counters: bool = False):
     if return_counters:
         return out, status, (cells, snake)
     return out, status
+
+
+def trace_pts(ts, paths, mode: int):
+    """Compute_Trace_PTS on the GPU for the A-view paths of a TaskSet's
+    alignments.  `paths` is the list la_batch returns (tasks whose status is
+    not 0 are skipped: pass their entry as None).  Returns, per task, None or
+    (diffs, int trace)."""
+    words, aw, bw = ts.packed()
+    idx = [i for i, p in enumerate(paths) if p is not None]
+    n = len(idx)
+    tasks = (MiTraceTask * max(n, 1))()
+    pts = []
+    off = 0
+    cap = 16
+    for q, i in enumerate(idx):
+        f, at, _ = paths[i]
+        t = tasks[q]
+        t.aword, t.bword = int(aw[i]), int(bw[i])
+        t.alen, t.blen = int(ts.par[i][0]), int(ts.par[i][1])
+        t.abpos, t.bbpos, t.aepos, t.bepos = f[0], f[1], f[2], f[3]
+        t.tlen = int(f[5])
+        t.points = off
+        t.selfie = 0
+        pts.append(np.asarray(at, dtype=np.uint16))
+        off += int(f[5])
+        cap += (f[2] - f[0]) + (f[3] - f[1]) + 16
+    points = np.concatenate(pts) if pts else np.zeros(1, dtype=np.uint16)
+    res = (MiTraceResult * max(n, 1))()
+    tr = np.zeros(cap, dtype=np.int32)
+    rc = lib.mi_trace_pts_host(ts.tspace, mode, words.ctypes.data, words.shape[0], points.ctypes.data, points.shape[0],
+                               tasks, n, res, tr.ctypes.data, cap)
+    if rc != 0:
+        raise RuntimeError(f"mi_trace_pts_host failed ({rc}): {lib.last_error()}")
+    out = [None] * len(paths)
+    for q, i in enumerate(idx):
+        r = res[q]
+        if r.status != 0:
+            raise RuntimeError(f"task {i}: trace status {r.status}")
+        out[i] = (r.diffs, tr[r.trace:r.trace + r.ntrace].copy())
+    return out

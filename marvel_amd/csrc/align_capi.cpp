@@ -16,6 +16,7 @@ using mi::SpecObj;
 struct WorkObj {
     std::vector<uint64_t> packed;
     std::vector<uint16_t> traces;
+    std::vector<int> itrace;   // Compute_Trace_* output
     Path bpath;
 };
 
@@ -127,6 +128,48 @@ Path* Local_Alignment(Alignment* align, Work_Data* work, Align_Spec* spec, int l
     bp->tlen = r.btlen;
     bp->trace = w->traces.data() + r.btrace;
     return bp;
+}
+
+int Compute_Trace_PTS(Alignment* align, Work_Data* work, int trace_spacing, int mode, int debug)
+{
+    (void)debug;
+    WorkObj* w = (WorkObj*)work;
+    Path* p = align->path;
+    const int alen = align->alen, blen = align->blen;
+    const long long aw = (alen + 31) / 32, bw = (blen + 31) / 32;
+    const bool selfie = (align->aseq == align->bseq);
+    const long long a0 = 1, b0 = selfie ? 1 : a0 + aw + 2;
+    w->packed.assign((selfie ? a0 + aw : b0 + bw) + 2, 0);
+    pack_into(w->packed, a0, align->aseq, alen);
+    if (!selfie) pack_into(w->packed, b0, align->bseq, blen);
+    MiTraceTask t;
+    t.aword = a0;
+    t.bword = b0;
+    t.alen = alen;
+    t.blen = blen;
+    t.abpos = p->abpos;
+    t.bbpos = p->bbpos;
+    t.aepos = p->aepos;
+    t.bepos = p->bepos;
+    t.tlen = p->tlen;
+    t.points = 0;
+    t.selfie = selfie ? 1 : 0;
+    const long long cap = (long long)(p->aepos - p->abpos) + (p->bepos - p->bbpos) + 16;
+    std::vector<int> out(cap);
+    MiTraceResult r;
+    const int rc = mi_trace_pts_host(trace_spacing, mode, w->packed.data(), (long long)w->packed.size(),
+                                     (const uint16_t*)p->trace, p->tlen, &t, 1, &r, out.data(), cap);
+    if (rc != 0) die("Compute_Trace_PTS failed");
+    if (r.status != 0) {
+        mi::mi_set_error("self comparison starts on diagonal 0");
+        die("Compute_Trace_PTS");
+    }
+    w->itrace.assign(out.begin(), out.begin() + r.ntrace);
+    if (w->itrace.empty()) w->itrace.push_back(0);
+    p->trace = w->itrace.data();
+    p->tlen = r.ntrace;
+    p->diffs = r.diffs;
+    return 0;
 }
 
 // ---- .las record helpers (dalign/align.c:3325-3449) -------------------------

@@ -421,6 +421,24 @@ __global__ void k_gather_start(const int* jobs, const long long* pstart, int n, 
     if (i < n) out[i] = pstart[jobs[i]];
 }
 
+// launch order of the jobs: most seeds first (the pairs with long true
+// overlaps), so the longest alignments do not start at the end of the queue.
+// key = (2^20 - 1 - min(seeds, 2^20 - 1)) << 32 | job, sorted on bits 32..51
+__global__ void k_job_keys(const int* jobs, const long long* pstart, int n, unsigned long long* keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long c = pstart[jobs[i] + 1] - pstart[jobs[i]];
+    const unsigned long long r = 0xFFFFFull - (unsigned long long)(c < 0xFFFFF ? c : 0xFFFFF);
+    keys[i] = (r << 32) | (unsigned)i;
+}
+
+__global__ void k_low32(const unsigned long long* keys, int n, int* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int)(keys[i] & 0xffffffffu);
+}
+
 template <int NCH>
 static void launch_report(const ReportArgs& a, int grid)
 {
@@ -570,7 +588,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     const int small_trace = TS <= TRACE_XOVR ? 1 : 0;
     const int tbytes = small_trace ? 1 : 2;
 
-    DBuf<int> bands, status, edge_in, edge_out, touch, order, counter;
+    DBuf<int> bands, status, edge_in, edge_out, touch, order, order_all, counter;
     DBuf<Peb> pools, pts;
     DBuf<uint16_t> latr, mtrace;
     DBuf<MPath> mpaths;
@@ -580,6 +598,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     edge_out.ensure(njobs);
     touch.ensure(njobs);
     order.ensure(njobs);
+    order_all.ensure(njobs);
     counter.ensure(1);
     ocount.ensure(2);
     stats.ensure(4);
@@ -638,9 +657,13 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     int cur_level = -1;
 
     // runs `todo` at `level`; returns the jobs that need another launch
+    // runs `todo` at `level` (todo empty: all jobs in the device launch order);
+    // returns the jobs that need another launch
     auto run = [&](std::vector<int>& todo, int level, bool with_edge_in) {
+        const bool all = todo.empty();
+        const long long nrun = all ? njobs : (long long)todo.size();
         const Level& L = levels[level];
-        const int grid = (int)std::min<long long>(L.waves, (long long)todo.size());
+        const int grid = (int)std::min<long long>(L.waves, nrun);
         if (level != cur_level) {
             bands.release();
             pools.release();
@@ -659,15 +682,15 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         mtrace.ensure((size_t)L.mtcap * gmax);
         HC(hipMemsetAsync(bands.p, 0, sizeof(int) * (size_t)(3 * w + 2) * gmax, st));
         HC(hipMemsetAsync(counter.p, 0, sizeof(int), st));
-        HC(hipMemcpyAsync(order.p, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice, st));
+        if (!all) HC(hipMemcpyAsync(order.p, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice, st));
         if (with_edge_in)
             HC(hipMemcpyAsync(edge_in.p, hedge_in.data(), sizeof(int) * njobs, hipMemcpyHostToDevice, st));
         u64 oc[2];
         HC(hipMemcpyAsync(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost, st));
         HC(hipStreamSynchronize(st));
         ReportArgs b = a;
-        b.order = order.p;
-        b.norder = (int)todo.size();
+        b.order = all ? order_all.p : order.p;
+        b.norder = (int)nrun;
         b.bands = bands.p;
         b.pools = pools.p;
         b.pool_cap = L.pool_cap;
@@ -705,6 +728,10 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         launches.push_back({(long long)oc[0], (long long)oc2[0]});
         std::vector<int> again;
         bool grow = false;
+        if (all) {
+            todo.resize(njobs);
+            for (long long j = 0; j < njobs; j++) todo[j] = (int)j;
+        }
         for (int j : todo) {
             const int s = hstatus[j];
             if (s == RP_OK) final_launch[j] = li;
@@ -741,9 +768,12 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
 
     // runs `todo` until every job is done: output-arena overflows are re-run
     // at the same window, wide bands / full pools at the next one
+    // todo empty: every job, in the device launch order
     auto run_all = [&](std::vector<int> todo, bool with_edge_in) {
         int level = 0;
-        while (!todo.empty()) {
+        bool first = true;
+        while (first || !todo.empty()) {
+            first = false;
             std::vector<int> again = run(todo, level, with_edge_in);
             S.retries += (long long)again.size();
             bool up = false, arena = false;
@@ -759,9 +789,15 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         }
     };
     {
-        std::vector<int> todo(njobs);
-        for (long long j = 0; j < njobs; j++) todo[j] = (int)j;
-        run_all(todo, false);
+        DBuf<u64> k1, k2;
+        k1.ensure(njobs);
+        k2.ensure(njobs);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_keys), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, (int)njobs, k1.p);
+        HC(hipGetLastError());
+        u64* sorted = radix_sort(k1.p, k2.p, njobs, 32, 52, KeyU64{});
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_low32), dim3(nblk(njobs)), dim3(256), 0, st, sorted, (int)njobs, order_all.p);
+        HC(hipGetLastError());
+        run_all(std::vector<int>(), false);
     }
 
     // the cell shared by lastp[maxdiag] and lasta[mindiag-1] carries from

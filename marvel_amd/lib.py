@@ -31,6 +31,16 @@ class MiLaResult(C.Structure):
                 ("snake", C.c_longlong)]
 
 
+class MiTraceTask(C.Structure):
+    _fields_ = [("aword", C.c_longlong), ("bword", C.c_longlong), ("alen", C.c_int), ("blen", C.c_int),
+                ("abpos", C.c_int), ("bbpos", C.c_int), ("aepos", C.c_int), ("bepos", C.c_int),
+                ("tlen", C.c_int), ("points", C.c_longlong), ("selfie", C.c_int)]
+
+
+class MiTraceResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("diffs", C.c_int), ("ntrace", C.c_int), ("trace", C.c_longlong)]
+
+
 # every symbol declared in include/marvl_align.h (checked by the CPU tests)
 EXPORTS = [
     "New_Align_Spec", "Free_Align_Spec", "Trace_Spacing", "Average_Correlation", "Base_Frequencies",
@@ -58,6 +68,10 @@ class _Lib:
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]
             h.mi_last_error.restype = C.c_char_p
             h.mi_device_count.restype = C.c_int
+            h.mi_trace_pts_host.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong,
+                                            C.POINTER(MiTraceTask), C.c_int, C.POINTER(MiTraceResult), C.c_void_p,
+                                            C.c_longlong]
+            h.mi_trace_pts_host.restype = C.c_int
             h.mi_selftest.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p,
                                       C.c_int, C.c_int, C.c_void_p, C.c_longlong]
             h.mi_selftest.restype = C.c_int

@@ -135,7 +135,7 @@ def main():
             for _ in range(args.warmup):
                 dal.compare_self(A, spec, 1, "C2.1", dev=dev, cA=cA)
             dal.synchronize()
-            times, stats = [], []
+            times, stats, phases = [], [], []
             for _ in range(args.steps):
                 dal.synchronize()
                 t = time.perf_counter()
@@ -143,6 +143,7 @@ def main():
                 dal.synchronize()
                 times.append(time.perf_counter() - t)
                 stats.append((s1, s2))
+                phases.append(dal.last_phases)
         finally:
             os.chdir(cwd)
         las = os.path.join(d, "001_00001", "C2.1.C2.1.las")
@@ -189,9 +190,10 @@ def main():
                              "ms_per_step": round(t_sort * 1e3, 2)},
             "cpu_baseline": cpu,
             "times_s": [round(x, 4) for x in times],
+            "phases_s": phases[-1],
             "setup_s": {"db_write": round(t_db, 2), "load_and_upload": round(t_load, 2)},
             "pass_stats": {k: [stats[-1][0][k], stats[-1][1][k]] for k in
-                           ("nhits", "npairs", "njobs", "naligned", "nrecords", "retries", "edge_events",
+                           ("nhits", "npairs", "njobs", "njobs_kept", "naligned", "nrecords", "retries", "edge_events",
                             "t_filter", "t_report", "t_copy", "t_total")},
         }
         print(json.dumps(line), flush=True)

@@ -101,6 +101,7 @@ typedef struct {
     long long report_launches;
     double t_seed_sort;             /* seconds in the seed radix sort (HIP events) */
     long long seed_sort_passes;     /* 8-bit digit passes of that sort */
+    long long njobs_kept;           /* pairs past the exact gate pre-check (the report kernel's jobs) */
 } MiFilterStats;
 
 /* Match_Filter returning an error code (MI_ERR_*) instead of exiting. */

@@ -57,10 +57,10 @@ static void print_stats(const char* what, double secs)
     MiFilterStats s;
     mi_filter_stats(&s);
     printf("{\"pass\": \"%s\", \"seconds\": %.4f, \"kmers_a\": %lld, \"kmers_b\": %lld, \"hits\": %lld, \"pairs\": %lld, "
-           "\"jobs\": %lld, \"aligned\": %lld, \"records\": %lld, \"cells\": %lld, \"snake\": %lld, \"limit\": %d, "
+           "\"jobs\": %lld, \"jobs_kept\": %lld, \"aligned\": %lld, \"records\": %lld, \"cells\": %lld, \"snake\": %lld, \"limit\": %d, "
            "\"edge_events\": %lld, \"edge_fixups\": %lld, \"retries\": %lld, \"t_filter\": %.4f, \"t_report\": %.4f, "
            "\"t_copy\": %.4f}\n",
-           what, secs, s.nkmers_a, s.nkmers_b, s.nhits, s.npairs, s.njobs, s.naligned, s.nrecords, s.cells, s.snake,
+           what, secs, s.nkmers_a, s.nkmers_b, s.nhits, s.npairs, s.njobs, s.njobs_kept, s.naligned, s.nrecords, s.cells, s.snake,
            s.limit, s.edge_events, s.edge_fixups, s.retries, s.t_filter, s.t_report, s.t_copy);
     fflush(stdout);
 }

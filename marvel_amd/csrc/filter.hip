@@ -22,7 +22,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -62,28 +64,78 @@ struct Fail : std::runtime_error {
         if (e_ != hipSuccess) throw Fail(std::string(#x) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
+// Device memory cache: released blocks go to a free list and are handed out
+// again for requests of up to their size (and at least half of it), so the
+// multi-GB buffers of a pass are mapped once per process, not once per call.
+// Sizes are rounded up to 2 MiB.
+struct DevCache {
+    std::mutex m;
+    std::multimap<size_t, void*> free_blocks;
+    void* get(size_t bytes, size_t* got)
+    {
+        const size_t r = (bytes + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+        {
+            std::lock_guard<std::mutex> g(m);
+            auto it = free_blocks.lower_bound(r);
+            if (it != free_blocks.end() && it->first <= 2 * r) {
+                void* p = it->second;
+                *got = it->first;
+                free_blocks.erase(it);
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, r) != hipSuccess) {
+            trim();   // give cached blocks back and try once more
+            HC(hipMalloc(&p, r));
+        }
+        *got = r;
+        return p;
+    }
+    void put(void* p, size_t bytes)
+    {
+        std::lock_guard<std::mutex> g(m);
+        free_blocks.emplace(bytes, p);
+    }
+    void trim()
+    {
+        std::lock_guard<std::mutex> g(m);
+        for (auto& kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+    }
+};
+static DevCache g_cache;
+
 template <class T>
 struct DBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0;        // elements usable
+    size_t bytes = 0;    // size of the cached block
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) g_cache.put(p, bytes);
         p = nullptr;
         n = 0;
+        bytes = 0;
     }
     T* ensure(size_t m)
     {
         if (m == 0) m = 1;
         if (m <= n) return p;
         release();
-        HC(hipMalloc(&p, m * sizeof(T)));
-        n = m;
+        p = (T*)g_cache.get(m * sizeof(T), &bytes);
+        n = bytes / sizeof(T);
         return p;
+    }
+    void swap(DBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(bytes, o.bytes);
     }
 };
 
@@ -248,7 +300,7 @@ static void build_keys(KIdx* x)
     }
     const int cs = x->fmt.cshift();
     uint64_t* sorted = radix_sort(x->keys.p, k2.p, nk, cs, cs + 2 * K, KeyU64{});
-    if (sorted != x->keys.p) std::swap(x->keys.p, k2.p);   // both buffers have the same size
+    if (sorted != x->keys.p) x->keys.swap(k2);
     x->nk = nk;
     if (g_suppress > 0 && nk > 0) {   // -t: drop groups of >= t entries (dalign/filter.c:645-696)
         DBuf<u64> flag, scan;
@@ -266,7 +318,7 @@ static void build_keys(KIdx* x)
         g_scan.run(flag.p, scan.p, nk);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact_u64), dim3(nblk(nk)), dim3(256), 0, st, x->keys.p, flag.p, scan.p, nk, k2.p);
         HC(hipGetLastError());
-        std::swap(x->keys.p, k2.p);
+        x->keys.swap(k2);
         x->nk = (long long)d2h1(scan.p + nk);
     }
     HC(hipStreamSynchronize(st));
@@ -424,12 +476,14 @@ __global__ void k_gather_start(const int* jobs, const long long* pstart, int n, 
 // launch order of the jobs: most seeds first (the pairs with long true
 // overlaps), so the longest alignments do not start at the end of the queue.
 // key = (2^20 - 1 - min(seeds, 2^20 - 1)) << 32 | job, sorted on bits 32..51
-__global__ void k_job_keys(const int* jobs, const long long* pstart, int n, unsigned long long* keys)
+__global__ void k_job_keys(const int* jobs, const long long* pstart, const unsigned long long* keep, int n,
+                           unsigned long long* keys)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long long c = pstart[jobs[i] + 1] - pstart[jobs[i]];
-    const unsigned long long r = 0xFFFFFull - (unsigned long long)(c < 0xFFFFF ? c : 0xFFFFF);
+    // kept jobs have c >= minhit >= 1, so their keys sort before the skipped ones (r = 2^20 - 1)
+    const unsigned long long r = keep[i] ? 0xFFFFFull - (unsigned long long)(c < 0xFFFFE ? c : 0xFFFFE) : 0xFFFFFull;
     keys[i] = (r << 32) | (unsigned)i;
 }
 
@@ -437,6 +491,66 @@ __global__ void k_low32(const unsigned long long* keys, int n, int* out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = (int)(keys[i] & 0xffffffffu);
+}
+
+// ---- record ordering on the GPU ----
+struct RecKey {
+    unsigned long long key;
+    long long idx;
+};
+struct RecKeyOf {
+    __device__ __forceinline__ unsigned long long operator()(const RecKey& r) const { return r.key; }
+};
+struct RecKeyFmt {
+    int rfirst, rb, pb, lb;   // read id base and bit widths of read ids, positions, list positions
+};
+
+// key (aread, bread, abpos, local) of the records a job's final launch wrote; all ones otherwise
+__global__ void k_rec_keys(const OutRec* __restrict__ r, long long n, int njobs, const int* __restrict__ final_launch,
+                           const long long* __restrict__ lb, const long long* __restrict__ le, RecKeyFmt f,
+                           RecKey* __restrict__ keys, unsigned long long* __restrict__ flag)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const OutRec x = r[i];
+    bool ok = x.job >= 0 && x.job < njobs;
+    if (ok) {
+        const int L = final_launch[x.job];
+        ok = L >= 0 && i >= lb[L] && i < le[L];
+    }
+    unsigned long long k = ~0ull;
+    if (ok)
+        k = ((unsigned long long)(x.aread - f.rfirst) << (f.rb + f.pb + f.lb)) |
+            ((unsigned long long)(x.bread - f.rfirst) << (f.pb + f.lb)) | ((unsigned long long)x.abpos << f.lb) |
+            (unsigned long long)x.local;
+    keys[i] = RecKey{k, i};
+    flag[i] = ok ? 1 : 0;
+}
+
+__global__ void k_rec_gather(const OutRec* __restrict__ r, const RecKey* __restrict__ sorted, long long n, int tbytes,
+                             OutRec* __restrict__ out, unsigned long long* __restrict__ tb)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const OutRec x = r[sorted[i].idx];
+    out[i] = x;
+    tb[i] = (unsigned long long)x.tlen * tbytes;
+}
+
+// one wave per record: its trace bytes to their place in the compacted array;
+// the record's toff becomes that place
+__global__ __launch_bounds__(64) void k_trace_gather(OutRec* __restrict__ recs, const uint8_t* __restrict__ src,
+                                                     const unsigned long long* __restrict__ pos, long long n,
+                                                     int tbytes, uint8_t* __restrict__ dst)
+{
+    const long long i = blockIdx.x;
+    if (i >= n) return;
+    const long long from = recs[i].toff;
+    const long long len = (long long)recs[i].tlen * tbytes;
+    uint8_t* d = dst + pos[i];
+    for (long long q = threadIdx.x; q < len; q += 64) d[q] = src[from + q];
+    __syncthreads();
+    if (threadIdx.x == 0) recs[i].toff = (long long)pos[i];
 }
 
 template <int NCH>
@@ -653,6 +767,8 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
 
     std::vector<int> hstatus(njobs, -1), final_launch(njobs, -1), hedge(njobs, 0), htouch(njobs, 0);
     std::vector<int> hedge_in(njobs, 0);
+    long long nfirst = 0;            // jobs of the first launch (those past the gate pre-check)
+    std::vector<int> first_order;    // and their launch order
     std::vector<std::pair<long long, long long>> launches;   // record index range of every launch
     int cur_level = -1;
 
@@ -661,9 +777,10 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     // returns the jobs that need another launch
     auto run = [&](std::vector<int>& todo, int level, bool with_edge_in) {
         const bool all = todo.empty();
-        const long long nrun = all ? njobs : (long long)todo.size();
+        const long long nrun = all ? nfirst : (long long)todo.size();
         const Level& L = levels[level];
         const int grid = (int)std::min<long long>(L.waves, nrun);
+        if (nrun == 0) return std::vector<int>();
         if (level != cur_level) {
             bands.release();
             pools.release();
@@ -728,10 +845,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         launches.push_back({(long long)oc[0], (long long)oc2[0]});
         std::vector<int> again;
         bool grow = false;
-        if (all) {
-            todo.resize(njobs);
-            for (long long j = 0; j < njobs; j++) todo[j] = (int)j;
-        }
+        if (all) todo = first_order;
         for (int j : todo) {
             const int s = hstatus[j];
             if (s == RP_OK) final_launch[j] = li;
@@ -756,10 +870,8 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
             HC(hipMemcpyAsync(r2.p, orecs.p, sizeof(OutRec) * ocap, hipMemcpyDeviceToDevice, st));
             HC(hipMemcpyAsync(t2.p, otr.p, otcap, hipMemcpyDeviceToDevice, st));
             HC(hipStreamSynchronize(st));
-            std::swap(orecs.p, r2.p);
-            std::swap(orecs.n, r2.n);
-            std::swap(otr.p, t2.p);
-            std::swap(otr.n, t2.n);
+            orecs.swap(r2);
+            otr.swap(t2);
             ocap = nrec;
             otcap = ntr;
         }
@@ -789,14 +901,35 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         }
     };
     {
-        DBuf<u64> k1, k2;
+        // exact gate pre-check: pairs whose seeds cannot pass report_thread's
+        // gate are finished here (no alignment, no record); the rest run
+        // longest-first
+        DBuf<u64> keep, kscan, k1, k2;
+        keep.ensure(njobs);
+        kscan.ensure(njobs + 1);
         k1.ensure(njobs);
         k2.ensure(njobs);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_keys), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, (int)njobs, k1.p);
+        HC(hipMemsetAsync(status.p, 0, sizeof(int) * njobs, st));
+        HC(hipMemsetAsync(edge_out.p, 0, sizeof(int) * njobs, st));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prefilter), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, (int)njobs,
+                           pstart.p, hits, K, BS, g_hitmin, minhit, mindiag, maxdiag, keep.p, touch.p);
+        HC(hipGetLastError());
+        g_scan.run(keep.p, kscan.p, njobs);
+        nfirst = (long long)d2h1(kscan.p + njobs);
+        S.njobs_kept += nfirst;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_keys), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, keep.p,
+                           (int)njobs, k1.p);
         HC(hipGetLastError());
         u64* sorted = radix_sort(k1.p, k2.p, njobs, 32, 52, KeyU64{});
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_low32), dim3(nblk(njobs)), dim3(256), 0, st, sorted, (int)njobs, order_all.p);
         HC(hipGetLastError());
+        first_order.resize(nfirst);
+        if (nfirst > 0)
+            HC(hipMemcpyAsync(first_order.data(), order_all.p, sizeof(int) * nfirst, hipMemcpyDeviceToHost, st));
+        HC(hipStreamSynchronize(st));
+        // skipped jobs are complete: status OK, the shared cell cleared when touched
+        HC(hipMemcpy(hstatus.data(), status.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(htouch.data(), touch.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
         run_all(std::vector<int>(), false);
     }
 
@@ -843,6 +976,11 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     S.t_report += t2 - t1;
 
     // ---- records back to the host ----
+    // The arena holds the records of every launch; a record is kept when it
+    // comes from its job's final launch.  The kept ones are ordered on the
+    // GPU by (aread, bread, abpos, position in the pair's list) -- the order
+    // report_thread's buffers give equal (aread, bread, COMP, abpos) keys --
+    // compacted with their traces, and copied back in one piece.
     u64 oc[2], sv[4];
     HC(hipMemcpy(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost));
     HC(hipMemcpy(sv, stats.p, sizeof sv, hipMemcpyDeviceToHost));
@@ -851,45 +989,66 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     S.snake += (long long)sv[2];
     S.edge_events += (long long)sv[3];
     const long long nrec = std::min<long long>((long long)oc[0], ocap);
-    std::vector<OutRec> recs(nrec);
-    if (nrec > 0) HC(hipMemcpy(recs.data(), orecs.p, sizeof(OutRec) * nrec, hipMemcpyDeviceToHost));
-    std::vector<uint8_t> tr(std::min<long long>((long long)oc[1], otcap));
-    if (!tr.empty()) HC(hipMemcpy(tr.data(), otr.p, tr.size(), hipMemcpyDeviceToHost));
-    std::vector<int> keep;
-    keep.reserve(nrec);
-    for (long long i = 0; i < nrec; i++) {
-        const OutRec& r = recs[i];
-        if (r.job < 0 || r.job >= njobs) continue;
-        const int L = final_launch[r.job];
-        if (L < 0 || i < launches[L].first || i >= launches[L].second) continue;
-        keep.push_back((int)i);
+    long long nkeep = 0, tbytes_all = 0;
+    DBuf<OutRec> srec;
+    DBuf<uint8_t> str;
+    if (nrec > 0) {
+        const int rfirst = std::min(A->ufirst, B->ufirst);
+        const int rlast = std::max(A->ufirst + A->nreads, B->ufirst + B->nreads);
+        const int rb = bits_for(std::max(1, rlast - rfirst));
+        const int pb = bits_for(std::max(A->maxlen, B->maxlen));
+        const int lb = bits_for(std::max(levels[0].mcap, levels[3].mcap));
+        if (2 * rb + pb + lb > 64) throw Fail("record keys exceed 64 bits", MI_ERR_ARG);
+        DBuf<int> dfinal;
+        DBuf<long long> dlb, dle;
+        dfinal.ensure(njobs);
+        dlb.ensure(launches.size());
+        dle.ensure(launches.size());
+        std::vector<long long> lbh(launches.size()), leh(launches.size());
+        for (size_t i = 0; i < launches.size(); i++) {
+            lbh[i] = launches[i].first;
+            leh[i] = launches[i].second;
+        }
+        HC(hipMemcpyAsync(dfinal.p, final_launch.data(), sizeof(int) * njobs, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(dlb.p, lbh.data(), sizeof(long long) * lbh.size(), hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(dle.p, leh.data(), sizeof(long long) * leh.size(), hipMemcpyHostToDevice, st));
+        DBuf<RecKey> k1, k2;
+        DBuf<u64> flag, pos;
+        k1.ensure(nrec);
+        k2.ensure(nrec);
+        flag.ensure(nrec);
+        pos.ensure(nrec + 1);
+        const RecKeyFmt kf{rfirst, rb, pb, lb};
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rec_keys), dim3(nblk(nrec)), dim3(256), 0, st, orecs.p, nrec, (int)njobs,
+                           dfinal.p, dlb.p, dle.p, kf, k1.p, flag.p);
+        HC(hipGetLastError());
+        g_scan.run(flag.p, pos.p, nrec);
+        nkeep = (long long)d2h1(pos.p + nrec);
+        RecKey* sorted = radix_sort(k1.p, k2.p, nrec, 0, 2 * rb + pb + lb, RecKeyOf{});
+        // sorted: the kept records first (the others have the all-ones key)
+        DBuf<u64> tb, tpos;
+        tb.ensure(std::max(nkeep, 1LL));
+        tpos.ensure(nkeep + 1);
+        srec.ensure(std::max(nkeep, 1LL));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rec_gather), dim3(nblk(nkeep)), dim3(256), 0, st, orecs.p, sorted, nkeep,
+                           tbytes, srec.p, tb.p);
+        HC(hipGetLastError());
+        g_scan.run(tb.p, tpos.p, nkeep);
+        tbytes_all = (long long)d2h1(tpos.p + nkeep);
+        str.ensure(std::max(tbytes_all, 1LL));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_gather), dim3(std::max<long long>(nkeep, 1)), dim3(64), 0, st, srec.p,
+                           otr.p, tpos.p, nkeep, tbytes, str.p);
+        HC(hipGetLastError());
     }
-    // order of equal (aread, bread, COMP, abpos) keys: the pair's list order
-    std::sort(keep.begin(), keep.end(), [&](int x, int y) {
-        const OutRec& l = recs[x];
-        const OutRec& r = recs[y];
-        if (l.aread != r.aread) return l.aread < r.aread;
-        if (l.bread != r.bread) return l.bread < r.bread;
-        if (l.abpos != r.abpos) return l.abpos < r.abpos;
-        return l.local < r.local;
-    });
-    Overlap_IO_Buffer* ob = spec->iobuf;
-    for (int i : keep) {
-        const OutRec& r = recs[i];
-        Overlap o;
-        o.path.tlen = r.tlen;
-        o.path.diffs = r.diffs;
-        o.path.abpos = r.abpos;
-        o.path.bbpos = r.bbpos;
-        o.path.aepos = r.aepos;
-        o.path.bepos = r.bepos;
-        o.path.trace = tr.data() + r.toff;
-        o.flags = r.flags;
-        o.aread = r.aread;
-        o.bread = r.bread;
-        if (AddOverlapToBuffer(ob, &o, tbytes)) throw Fail("cannot grow the overlap buffer");
-    }
-    S.nrecords += (long long)keep.size();
+    std::vector<OutRec> recs(nkeep);
+    std::vector<uint8_t> tr(tbytes_all);
+    if (nkeep > 0) HC(hipMemcpyAsync(recs.data(), srec.p, sizeof(OutRec) * nkeep, hipMemcpyDeviceToHost, st));
+    if (tbytes_all > 0) HC(hipMemcpyAsync(tr.data(), str.p, tbytes_all, hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    // srec[i].toff now holds the offset in tr
+    if (mi_append_records(spec->iobuf, recs.data(), nkeep, tr.data(), tbytes_all, tbytes) != 0)
+        throw Fail("cannot grow the overlap buffer");
+    S.nrecords += nkeep;
     S.t_copy += now_s() - t2;
 }
 

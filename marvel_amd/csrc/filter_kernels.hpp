@@ -322,3 +322,84 @@ __global__ void k_compact_idx(const unsigned long long* __restrict__ sel, const 
 }
 
 }  // namespace mi
+
+namespace mi {
+
+// Exact pre-check of a read pair against report_thread's gate
+// (dalign/filter.c:1740-1778): a pair can only align a seed if, in some
+// panel with at least minhit seeds, a seed's band score plus a neighbour
+// band's score reaches hitmin.  Scores are computed as the report kernel
+// computes them, with the band arrays' edge cells (score[maxdiag+1] is
+// lastp[mindiag], score[mindiag-1] is the zero pad).  lasta only makes the
+// gate harder and is 0 when a pair starts, and a non-zero carried
+// lastp[maxdiag] only lowers scores, so a pair that fails here aligns
+// nothing.  Pairs with more than kPreMax seeds are always kept.
+// Per job: keep (1: run the report kernel), touch (1: some scored panel has
+// a seed in band maxdiag, i.e. the pair reads and clears the shared cell).
+constexpr int kPreMax = 32;
+
+__global__ void k_prefilter(const int* __restrict__ jobs, int njobs, const long long* __restrict__ pstart,
+                            const Seed* __restrict__ h, int K, int binshift, int hitmin, int minhit, int mindiag,
+                            int maxdiag, unsigned long long* __restrict__ keep, int* __restrict__ touch)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const int p = jobs[j];
+    const long long s = pstart[p], e = pstart[p + 1];
+    if (e - s > kPreMax) {
+        keep[j] = 1;
+        touch[j] = 0;
+        return;
+    }
+    int band[kPreMax], contrib[kPreMax];
+    bool pass = false, tch = false;
+    long long nidx = s;
+    int amark2 = 0;
+    while (nidx < e && !pass) {
+        const int amark = amark2 + 50000;
+        amark2 = amark - 10000;
+        const long long lidx = nidx;
+        long long h2 = lidx;
+        int apos;
+        do {
+            apos = h[nidx].apos;
+            nidx++;
+            if (apos <= amark2) h2 = nidx;
+        } while (nidx < e && apos <= amark);
+        const int n = (int)(nidx - lidx);
+        if (n >= minhit) {
+            int lastp_min = 0;
+            for (int f = 0; f < n; f++) {
+                const Seed x = h[lidx + f];
+                band[f] = x.diag >> binshift;
+                int prev = 0;
+                for (int g = f - 1; g >= 0; g--)
+                    if (band[g] == band[f]) {
+                        prev = h[lidx + g].apos;
+                        break;
+                    }
+                const int d = x.apos - prev;
+                contrib[f] = d >= K ? K : d;
+                if (band[f] == mindiag) lastp_min = x.apos;
+                if (band[f] == maxdiag) tch = true;
+            }
+            auto S = [&](int b) -> int {
+                if (b == maxdiag + 1) return lastp_min;
+                if (b == mindiag - 1) return 0;
+                int t = 0;
+                for (int g = 0; g < n; g++)
+                    if (band[g] == b) t += contrib[g];
+                return t;
+            };
+            for (int f = 0; f < n && !pass; f++) {
+                const int sb = S(band[f]);
+                if (sb + S(band[f] + 1) >= hitmin || sb + S(band[f] - 1) >= hitmin) pass = true;
+            }
+        }
+        nidx = h2;
+    }
+    keep[j] = pass ? 1 : 0;
+    touch[j] = tch ? 1 : 0;
+}
+
+}  // namespace mi

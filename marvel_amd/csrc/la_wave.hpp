@@ -116,6 +116,18 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 __device__ __forceinline__ int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+// value of lane (L-1) & 63 / (L+1) & 63: DPP wave_ror:1 / wave_rol:1, full-rate
+// VALU moves (no LDS traffic).  All 64 lanes must be active.
+__device__ __forceinline__ int lane_below(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x13C, 0xF, 0xF, false); }
+__device__ __forceinline__ int lane_above(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ uint64_t lane_below64(uint64_t v)
+{
+    return ((uint64_t)(uint32_t)lane_below((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)lane_below((int)(uint32_t)v);
+}
+__device__ __forceinline__ uint64_t lane_above64(uint64_t v)
+{
+    return ((uint64_t)(uint32_t)lane_above((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)lane_above((int)(uint32_t)v);
+}
 __device__ __forceinline__ uint64_t rdl64(uint64_t v, int lane)
 {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
@@ -160,6 +172,22 @@ __device__ __forceinline__ Diag diag_shfl(const Diag& D, int sl)
     g.T = shfl_u64(D.T, sl);
     g.HA = shfl_i(D.HA, sl); g.HAm = shfl_i(D.HAm, sl); g.HAd = shfl_i(D.HAd, sl);
     g.HB = shfl_i(D.HB, sl); g.HBm = shfl_i(D.HBm, sl); g.HBd = shfl_i(D.HBd, sl);
+    return g;
+}
+
+// fields of lane L + sel (sel in {-1, 0, +1}, wrapping like (L + sel) & 63)
+__device__ __forceinline__ int pick3(int v, int sel)
+{
+    const int b = lane_below(v), a = lane_above(v);
+    return sel < 0 ? b : (sel > 0 ? a : v);
+}
+__device__ __forceinline__ Diag diag_pick(const Diag& D, int sel)
+{
+    Diag g;
+    g.V = pick3(D.V, sel); g.M = pick3(D.M, sel); g.NA = pick3(D.NA, sel); g.NB = pick3(D.NB, sel);
+    g.T = ((uint64_t)(uint32_t)pick3((int)(uint32_t)(D.T >> 32), sel) << 32) | (uint32_t)pick3((int)(uint32_t)D.T, sel);
+    g.HA = pick3(D.HA, sel); g.HAm = pick3(D.HAm, sel); g.HAd = pick3(D.HAd, sel);
+    g.HB = pick3(D.HB, sel); g.HBm = pick3(D.HBm, sel); g.HBd = pick3(D.HBd, sel);
     return g;
 }
 
@@ -561,8 +589,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         int Vm[NCH], Vp[NCH];
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
-            int up = shfl_i(bd[j].V, (lane + 63) & 63);
-            int dn = shfl_i(bd[j].V, (lane + 1) & 63);
+            int up = lane_below(bd[j].V);
+            int dn = lane_above(bd[j].V);
             const int upc = (j > 0) ? rdl(bd[j - 1].V, 63) : SENT;       // wave-uniform reads
             const int dnc = (j < NCH - 1) ? rdl(bd[j + 1].V, 0) : SENT;
             if (lane == 0) up = upc;
@@ -574,8 +602,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
             const int k = base + 64 * j + lane;
-            int naU = shfl_i(bd[j].NA, (lane + 63) & 63), nbU = shfl_i(bd[j].NB, (lane + 63) & 63);
-            int naD = shfl_i(bd[j].NA, (lane + 1) & 63), nbD = shfl_i(bd[j].NB, (lane + 1) & 63);
+            int naU = lane_below(bd[j].NA), nbU = lane_below(bd[j].NB);
+            int naD = lane_above(bd[j].NA), nbD = lane_above(bd[j].NB);
             if (j > 0) {
                 const int a = rdl(bd[j - 1].NA, 63), b = rdl(bd[j - 1].NB, 63);
                 if (lane == 0) { naU = a; nbU = b; }
@@ -605,7 +633,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 else if (ac > ap || ac > am) sel = -1;
                 else sel = 0;
             }
-            Diag g = diag_shfl(bd[j], (lane + sel) & 63);
+            Diag g = diag_pick(bd[j], sel);
             if (j > 0) {
                 const Diag e = diag_rdl(bd[j - 1], 63);
                 if (lane == 0 && sel == -1) g = e;

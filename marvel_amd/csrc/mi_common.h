@@ -33,6 +33,11 @@ struct SpecObj {
 };
 
 void spec_build(HostSpec* s, double ave_corr, int tspace, const float* freq);
+struct OutRec;
+// Appends n records (their trace bytes at tr + toff) to an overlap buffer in
+// one piece; the behaviour of n AddOverlapToBuffer calls.  Returns 0.
+int mi_append_records(Overlap_IO_Buffer* b, const OutRec* recs, long long n, const uint8_t* tr, long long trbytes,
+                      int tbytes);
 // Device tables for these parameters (cached per parameter set; lives until exit).
 DevSpecHost* mi_spec_device(const MiAlignParams* prm);
 int mi_num_cus();

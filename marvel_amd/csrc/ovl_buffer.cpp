@@ -12,6 +12,7 @@
 
 #include "../../include/marvl_align.h"
 #include "mi_common.h"
+#include "records.h"
 
 using mi::SpecObj;
 
@@ -74,6 +75,63 @@ bool write_las(const std::string& path, const std::vector<const Overlap*>& recs,
 
 }  // namespace
 
+namespace mi {
+
+int mi_append_records(Overlap_IO_Buffer* b, const OutRec* recs, long long n, const uint8_t* tr, long long trbytes,
+                      int tbytes)
+{
+    if (n <= 0) return 0;
+    if (b->otop + n > b->omax) {
+        const long long nmax = (long long)((b->otop + n) * 1.2) + 1000;
+        if (nmax > INT32_MAX) return 1;
+        Overlap* o = (Overlap*)realloc(b->ovls, sizeof(Overlap) * nmax);
+        if (o == nullptr) return 1;
+        b->ovls = o;
+        b->omax = (int)nmax;
+    }
+    const bool with_trace = b->no_trace == 0;
+    const uint64_t need = with_trace ? (uint64_t)trbytes : 0;
+    if (with_trace && b->ttop + need > b->tmax) {
+        const uint64_t nmax = (uint64_t)((b->ttop + need) * 1.2) + 1000;
+        char* t = (char*)realloc(b->trace, nmax);
+        if (t == nullptr) return 1;
+        if (t != b->trace) {   // keep the buffered records' trace pointers valid
+            for (int i = 0; i < b->otop; i++)
+                if (b->ovls[i].path.trace != nullptr)
+                    b->ovls[i].path.trace = t + ((char*)b->ovls[i].path.trace - (char*)b->trace);
+        }
+        b->trace = t;
+        b->tmax = nmax;
+    }
+    char* tbase = (char*)b->trace + b->ttop;
+    if (with_trace && need) memcpy(tbase, tr, need);
+    Overlap* o = b->ovls + b->otop;
+    for (long long i = 0; i < n; i++) {
+        const OutRec& r = recs[i];
+        o[i].path.diffs = r.diffs;
+        o[i].path.abpos = r.abpos;
+        o[i].path.bbpos = r.bbpos;
+        o[i].path.aepos = r.aepos;
+        o[i].path.bepos = r.bepos;
+        o[i].flags = r.flags;
+        o[i].aread = r.aread;
+        o[i].bread = r.bread;
+        if (with_trace) {
+            o[i].path.tlen = r.tlen;
+            o[i].path.trace = tbase + r.toff;
+        } else {
+            o[i].path.tlen = 0;
+            o[i].path.trace = nullptr;
+        }
+    }
+    (void)tbytes;
+    b->otop += (int)n;
+    b->ttop += need;
+    return 0;
+}
+
+}  // namespace mi
+
 extern "C" {
 
 Overlap_IO_Buffer* CreateOverlapBuffer(int nthreads, int tbytes, int no_trace)
@@ -123,13 +181,10 @@ int AddOverlapToBuffer(Overlap_IO_Buffer* b, Overlap* ovl, int tbytes)
         const uint64_t nmax = (uint64_t)((b->ttop + tb) * 1.2) + 1000;
         char* t = (char*)realloc(b->trace, nmax);
         if (t == nullptr) return 1;
-        if (t != b->trace) {
-            uint64_t off = 0;
+        if (t != b->trace) {   // keep the buffered records' trace pointers valid
             for (int i = 0; i < b->otop; i++)
-                if (b->ovls[i].path.trace != nullptr) {
-                    b->ovls[i].path.trace = t + off;
-                    off += (uint64_t)b->ovls[i].path.tlen * tbytes;
-                }
+                if (b->ovls[i].path.trace != nullptr)
+                    b->ovls[i].path.trace = t + ((char*)b->ovls[i].path.trace - (char*)b->trace);
         }
         b->trace = t;
         b->tmax = nmax;
@@ -174,12 +229,24 @@ void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblo
             const Overlap* o = s->iobuf[i].ovls + j;
             if (!s->only_identity || o->aread == o->bread) all.push_back(o);
         }
-    std::stable_sort(all.begin(), all.end(), [](const Overlap* l, const Overlap* r) {
+    auto less = [](const Overlap* l, const Overlap* r) {
         if (l->aread != r->aread) return l->aread < r->aread;
         if (l->bread != r->bread) return l->bread < r->bread;
         if (COMP(l->flags) != COMP(r->flags)) return COMP(l->flags) < COMP(r->flags);
         return l->path.abpos < r->path.abpos;
-    });
+    };
+    // the buffers usually hold a few already sorted runs (one per filter
+    // pass): merge them stably; anything else gets a full stable sort
+    std::vector<size_t> runs{0};
+    for (size_t i = 1; i < all.size(); i++)
+        if (less(all[i], all[i - 1])) runs.push_back(i);
+    if (runs.size() <= 16) {
+        for (size_t r = 1; r < runs.size(); r++) {
+            const size_t end = (r + 1 < runs.size()) ? runs[r + 1] : all.size();
+            std::inplace_merge(all.begin(), all.begin() + runs[r], all.begin() + end, less);
+        }
+    } else
+        std::stable_sort(all.begin(), all.end(), less);
     std::string aroot, broot;
     const int aid = split_block_name(ablock, aroot);
     const int bid = split_block_name(bblock, broot);

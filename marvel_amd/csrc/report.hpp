@@ -25,6 +25,7 @@
 
 #include "filter_kernels.hpp"
 #include "la_wave.hpp"
+#include "records.h"
 
 namespace mi {
 
@@ -40,14 +41,6 @@ struct MPath {   // a kept alignment of the pair (dalign/align.h:76-83 fields)
     long long toff;   // offset of its u16 trace in the wave's trace pool
 };
 
-struct OutRec {   // one .las record plus its sort tie-breaker
-    int tlen, diffs, abpos, bbpos, aepos, bepos;
-    uint32_t flags;
-    int aread, bread;
-    int local;        // position in the pair's kept list (ties of the final sort)
-    int job;          // job that wrote the record (-1: slot never written)
-    long long toff;   // offset of the trace bytes in the output trace arena
-};
 
 struct ReportArgs {
     const Seed* hits;

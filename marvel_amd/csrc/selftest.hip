@@ -66,6 +66,15 @@ __global__ __launch_bounds__(64) void st_match(const uint64_t* A, const uint64_t
     out[2 * i + 1] = mism ? 1 : 0;
 }
 
+// test 3: DPP lane moves; per lane: [lane_below(v), lane_above(v)]
+__global__ __launch_bounds__(64) void st_dpp(const int* in, int* out)
+{
+    const int lane = __lane_id();
+    const int v = in[blockIdx.x * 64 + lane];
+    out[(blockIdx.x * 64 + lane) * 2] = lane_below(v);
+    out[(blockIdx.x * 64 + lane) * 2 + 1] = lane_above(v);
+}
+
 }  // namespace mi
 
 using namespace mi;
@@ -93,6 +102,9 @@ extern "C" int mi_selftest(int which, const void* in, long long nin, const void*
     case 2:
         hipLaunchKernelGGL(HIP_KERNEL_NAME(st_match), dim3((nq + 63) / 64), dim3(64), 0, 0, (const uint64_t*)din + 1, (const uint64_t*)din2 + 1, (const int*)dq, nq,
                                          (int*)dout);
+        break;
+    case 3:
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(st_dpp), dim3((int)(nin / 256)), dim3(64), 0, 0, (const int*)din, (int*)dout);
         break;
     default:
         rc = MI_ERR_ARG;

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 
 from .lib import LIB_PATH, LibraryMissing
 
@@ -44,7 +45,7 @@ class MiFilterStats(C.Structure):
                 ("edge_events", C.c_longlong), ("edge_fixups", C.c_longlong), ("retries", C.c_longlong),
                 ("limit", C.c_int), ("t_filter", C.c_double), ("t_report", C.c_double), ("t_copy", C.c_double),
                 ("t_total", C.c_double), ("t_report_kernel", C.c_double), ("report_launches", C.c_longlong),
-                ("t_seed_sort", C.c_double), ("seed_sort_passes", C.c_longlong)]
+                ("t_seed_sort", C.c_double), ("seed_sort_passes", C.c_longlong), ("njobs_kept", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -176,14 +177,22 @@ class Daligner:
         already in HBM.  Returns (forward stats, complement stats)."""
         nm = C.create_string_buffer(name.encode())
         os.makedirs("%03d_%05d" % (run_id, A.part), exist_ok=True)   # createSubdir (dalign/daligner.c:583-611)
+        t = [time.perf_counter()]
         a = self.sort_kmers_device(dev) if dev is not None else self.sort_kmers(A)
+        t.append(time.perf_counter())
         try:
             s1 = self.match_filter(nm, A, nm, A, a, a, 0, spec)
+            t.append(time.perf_counter())
             cA = cA if cA is not None else self.complement_header(A)
             c = self.sort_kmers_complement(a, cA)
+            t.append(time.perf_counter())
             s2 = self.match_filter(nm, A, nm, cA, a, c, 1, spec)   # consumes c
+            t.append(time.perf_counter())
         finally:
             self.free_index(a)
         self.write_overlap_buffer(spec, run_id, nm, nm, A.ufirst + A.nreads - 1)
         self.reset_overlap_buffer(spec)
+        t.append(time.perf_counter())
+        self.last_phases = {k: round(t[i + 1] - t[i], 4) for i, k in
+                            enumerate(("index", "forward", "complement_index", "complement", "write"))}
         return s1, s2

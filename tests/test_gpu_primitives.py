@@ -77,3 +77,16 @@ def test_snake_matcher():
             while s < lim and a[x - 1 - s] == b[y - 1 - s]:
                 s += 1
         assert out[i, 0] == s and out[i, 1] == int(s < lim), (i, x, y, lim, d, out[i], s)
+
+
+def test_dpp_wave_rotations():
+    """lane_below / lane_above (DPP wave_ror:1 / wave_rol:1) read lanes L-1 / L+1 mod 64."""
+    print("[prim] dpp", flush=True)
+    rng = np.random.default_rng(9)
+    waves = 8
+    v = rng.integers(-10**6, 10**6, size=waves * 64).astype(np.int32)
+    out = _run(3, inp=v, nout=waves * 64 * 2).reshape(waves, 64, 2)
+    for w in range(waves):
+        x = v[w * 64:(w + 1) * 64]
+        assert np.array_equal(out[w, :, 0], np.roll(x, 1))
+        assert np.array_equal(out[w, :, 1], np.roll(x, -1))

@@ -86,6 +86,18 @@ inline unsigned long long __ballot(int p)
 }
 inline int __any(int p) { return __ballot(p) != 0; }
 inline int __builtin_amdgcn_readlane(int v, int l) { return emu::xchg(v, l); }
+// DPP wave moves: 0x13C wave_ror:1, 0x134 wave_rol:1, 0x138 wave_shr:1, 0x130 wave_shl:1
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, bool)
+{
+    const int l = emu::lane;
+    switch (ctrl) {
+    case 0x13C: return emu::xchg(src, (l + 63) & 63);
+    case 0x134: return emu::xchg(src, (l + 1) & 63);
+    case 0x138: { const int v = emu::xchg(src, (l + 63) & 63); return l == 0 ? old : v; }
+    case 0x130: { const int v = emu::xchg(src, (l + 1) & 63); return l == 63 ? old : v; }
+    default: __builtin_trap();
+    }
+}
 inline int __builtin_amdgcn_readfirstlane(int v) { return emu::xchg(v, 0); }
 inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned acc)
 {

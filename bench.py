@@ -102,7 +102,19 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 20
+        x = torch.ones(n, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "read": n, "written": n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "block pairs shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
@@ -194,6 +206,7 @@ def main():
             "setup_s": {"db_write": round(t_db, 2), "load_and_upload": round(t_load, 2)},
             "pass_stats": {k: [stats[-1][0][k], stats[-1][1][k]] for k in
                            ("nhits", "npairs", "njobs", "njobs_kept", "naligned", "nrecords", "retries", "edge_events",
+                            "cells", "snake", "wave_steps",
                             "t_filter", "t_report", "t_copy", "t_total")},
         }
         print(json.dumps(line), flush=True)

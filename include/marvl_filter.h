@@ -102,6 +102,7 @@ typedef struct {
     double t_seed_sort;             /* seconds in the seed radix sort (HIP events) */
     long long seed_sort_passes;     /* 8-bit digit passes of that sort */
     long long njobs_kept;           /* pairs past the exact gate pre-check (the report kernel's jobs) */
+    long long wave_steps;           /* aligner wave steps (one per difference level and pass) */
 } MiFilterStats;
 
 /* Match_Filter returning an error code (MI_ERR_*) instead of exiting. */

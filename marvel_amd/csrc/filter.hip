@@ -493,6 +493,50 @@ __global__ void k_low32(const unsigned long long* keys, int n, int* out)
     if (i < n) out[i] = (int)(keys[i] & 0xffffffffu);
 }
 
+// (job, status) of the jobs of a launch that did not finish
+__global__ void k_collect_failed(const int* __restrict__ order, long long n, const int* __restrict__ status,
+                                 int* __restrict__ flist, int* __restrict__ fcount)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = order[i];
+    const int s = status[j];
+    if (s != 0) {
+        const int q = atomicAdd(fcount, 1);
+        flist[2 * q] = j;
+        flist[2 * q + 1] = s;
+    }
+}
+
+__global__ void k_flag_nonzero(const int* __restrict__ v, long long n, unsigned long long* __restrict__ flag)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = v[i] != 0 ? 1 : 0;
+}
+
+__global__ void k_touched_info(const int* __restrict__ tj, long long n, const int* __restrict__ jobs,
+                               const long long* __restrict__ pstart, const int* __restrict__ edge_out,
+                               int* __restrict__ te, long long* __restrict__ ts)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = tj[i];
+    te[i] = edge_out[j];
+    ts[i] = pstart[jobs[j]];
+}
+
+__global__ void k_scatter_int(const int* __restrict__ idx, const int* __restrict__ v, long long n, int* __restrict__ dst)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = v[i];
+}
+
+__global__ void k_gather_int(const int* __restrict__ idx, const int* __restrict__ src, long long n, int* __restrict__ dst)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
 // ---- record ordering on the GPU ----
 struct RecKey {
     unsigned long long key;
@@ -506,18 +550,13 @@ struct RecKeyFmt {
 };
 
 // key (aread, bread, abpos, local) of the records a job's final launch wrote; all ones otherwise
-__global__ void k_rec_keys(const OutRec* __restrict__ r, long long n, int njobs, const int* __restrict__ final_launch,
-                           const long long* __restrict__ lb, const long long* __restrict__ le, RecKeyFmt f,
-                           RecKey* __restrict__ keys, unsigned long long* __restrict__ flag)
+__global__ void k_rec_keys(const OutRec* __restrict__ r, long long n, int njobs, const int* __restrict__ final_gen,
+                           RecKeyFmt f, RecKey* __restrict__ keys, unsigned long long* __restrict__ flag)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const OutRec x = r[i];
-    bool ok = x.job >= 0 && x.job < njobs;
-    if (ok) {
-        const int L = final_launch[x.job];
-        ok = L >= 0 && i >= lb[L] && i < le[L];
-    }
+    const bool ok = x.job >= 0 && x.job < njobs && final_gen[x.job] == x.gen;
     unsigned long long k = ~0ull;
     if (ok)
         k = ((unsigned long long)(x.aread - f.rfirst) << (f.rb + f.pb + f.lb)) |
@@ -675,12 +714,6 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     DBuf<int> jobs;
     jobs.ensure(njobs);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact_idx), dim3(nblk(np)), dim3(256), 0, st, sel.p, sscan.p, np, jobs.p);
-    DBuf<long long> jstart;
-    jstart.ensure(njobs);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_start), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, (int)njobs, jstart.p);
-    HC(hipGetLastError());
-    std::vector<long long> hjstart(njobs);
-    HC(hipMemcpyAsync(hjstart.data(), jstart.p, sizeof(long long) * njobs, hipMemcpyDeviceToHost, st));
     HC(hipStreamSynchronize(st));
     const double t1 = now_s();
     S.t_filter += t1 - t0;
@@ -702,25 +735,57 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     const int small_trace = TS <= TRACE_XOVR ? 1 : 0;
     const int tbytes = small_trace ? 1 : 2;
 
-    DBuf<int> bands, status, edge_in, edge_out, touch, order, order_all, counter;
+    DBuf<int> bands, status, final_gen, edge_in, edge_out, touch, order, order_all, counter, flist, fcount;
     DBuf<Peb> pools, pts;
     DBuf<uint16_t> latr, mtrace;
     DBuf<MPath> mpaths;
     DBuf<u64> ocount, stats;
     status.ensure(njobs);
+    final_gen.ensure(njobs);
     edge_in.ensure(njobs);
     edge_out.ensure(njobs);
     touch.ensure(njobs);
     order.ensure(njobs);
     order_all.ensure(njobs);
+    flist.ensure(njobs);
     counter.ensure(1);
+    fcount.ensure(1);
     ocount.ensure(2);
-    stats.ensure(4);
+    stats.ensure(5);
     HC(hipMemsetAsync(ocount.p, 0, 2 * sizeof(u64), st));
-    HC(hipMemsetAsync(stats.p, 0, 4 * sizeof(u64), st));
+    HC(hipMemsetAsync(stats.p, 0, 5 * sizeof(u64), st));
     HC(hipMemsetAsync(edge_in.p, 0, sizeof(int) * njobs, st));
+    HC(hipMemsetAsync(edge_out.p, 0, sizeof(int) * njobs, st));
+    HC(hipMemsetAsync(status.p, 0, sizeof(int) * njobs, st));
+    HC(hipMemsetAsync(final_gen.p, 0xff, sizeof(int) * njobs, st));
 
-    long long ocap = std::max<long long>(4096, 2 * njobs);
+    // exact gate pre-check: pairs whose seeds cannot pass report_thread's
+    // gate are finished here (no alignment, no record); the rest run
+    // longest-first
+    long long nfirst = 0;
+    {
+        DBuf<u64> keep, kscan, k1, k2;
+        keep.ensure(njobs);
+        kscan.ensure(njobs + 1);
+        k1.ensure(njobs);
+        k2.ensure(njobs);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prefilter), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, (int)njobs,
+                           pstart.p, hits, K, BS, g_hitmin, minhit, mindiag, maxdiag, keep.p, touch.p);
+        HC(hipGetLastError());
+        g_scan.run(keep.p, kscan.p, njobs);
+        nfirst = (long long)d2h1(kscan.p + njobs);
+        S.njobs_kept += nfirst;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_keys), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, keep.p,
+                           (int)njobs, k1.p);
+        HC(hipGetLastError());
+        u64* sorted = radix_sort(k1.p, k2.p, njobs, 32, 52, KeyU64{});
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_low32), dim3(nblk(njobs)), dim3(256), 0, st, sorted, (int)njobs,
+                           order_all.p);
+        HC(hipGetLastError());
+    }
+
+    // output arenas: about two records per kept pair; overflows grow them
+    long long ocap = std::max<long long>(4096, 4 * nfirst);
     long long otcap = ocap * 160;
     DBuf<OutRec> orecs;
     DBuf<uint8_t> otr;
@@ -757,30 +822,25 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     a.w = w;
     a.sp = ds->spec;
     a.maxtp_all = maxtp_all;
+    a.edge_in = edge_in.p;
     a.edge_out = edge_out.p;
     a.touch = touch.p;
     a.status = status.p;
+    a.final_gen = final_gen.p;
     a.counter = counter.p;
     a.abort_flag = mi_abort_flag_device();
     a.stats = stats.p;
     a.ocount = ocount.p;
 
-    std::vector<int> hstatus(njobs, -1), final_launch(njobs, -1), hedge(njobs, 0), htouch(njobs, 0);
-    std::vector<int> hedge_in(njobs, 0);
-    long long nfirst = 0;            // jobs of the first launch (those past the gate pre-check)
-    std::vector<int> first_order;    // and their launch order
-    std::vector<std::pair<long long, long long>> launches;   // record index range of every launch
-    int cur_level = -1;
+    int cur_level = -1, gen = 0;
 
-    // runs `todo` at `level`; returns the jobs that need another launch
-    // runs `todo` at `level` (todo empty: all jobs in the device launch order);
-    // returns the jobs that need another launch
-    auto run = [&](std::vector<int>& todo, int level, bool with_edge_in) {
-        const bool all = todo.empty();
-        const long long nrun = all ? nfirst : (long long)todo.size();
+    // runs the n jobs listed at dorder (device) at `level`; returns the
+    // (job, status) pairs that need another launch
+    auto run = [&](const int* dorder, long long n, int level) {
+        std::vector<std::pair<int, int>> again;
+        if (n == 0) return again;
         const Level& L = levels[level];
-        const int grid = (int)std::min<long long>(L.waves, nrun);
-        if (nrun == 0) return std::vector<int>();
+        const int grid = (int)std::min<long long>(L.waves, n);
         if (level != cur_level) {
             bands.release();
             pools.release();
@@ -799,15 +859,11 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         mtrace.ensure((size_t)L.mtcap * gmax);
         HC(hipMemsetAsync(bands.p, 0, sizeof(int) * (size_t)(3 * w + 2) * gmax, st));
         HC(hipMemsetAsync(counter.p, 0, sizeof(int), st));
-        if (!all) HC(hipMemcpyAsync(order.p, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice, st));
-        if (with_edge_in)
-            HC(hipMemcpyAsync(edge_in.p, hedge_in.data(), sizeof(int) * njobs, hipMemcpyHostToDevice, st));
-        u64 oc[2];
-        HC(hipMemcpyAsync(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost, st));
-        HC(hipStreamSynchronize(st));
+        HC(hipMemsetAsync(fcount.p, 0, sizeof(int), st));
         ReportArgs b = a;
-        b.order = all ? order_all.p : order.p;
-        b.norder = (int)nrun;
+        b.order = dorder;
+        b.norder = (int)n;
+        b.gen = gen++;
         b.bands = bands.p;
         b.pools = pools.p;
         b.pool_cap = L.pool_cap;
@@ -818,7 +874,6 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         b.mcap = L.mcap;
         b.mtrace = mtrace.p;
         b.mtcap = L.mtcap;
-        b.edge_in = edge_in.p;
         b.orecs = orecs.p;
         b.ocap = ocap;
         b.otr = otr.p;
@@ -831,33 +886,33 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         }
         HC(hipGetLastError());
         g_ev_report.record_stop();
+        // jobs of this launch that did not finish
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collect_failed), dim3(nblk(n)), dim3(256), 0, st, dorder, n, status.p,
+                           flist.p, fcount.p);
         const int wd = mi_wait_watchdog(st, "Match_Filter report kernel");
         if (wd > 0) throw Fail("report kernel exceeded the watchdog deadline and was stopped", MI_ERR_TIMEOUT);
         if (wd < 0) throw Fail("report kernel failed on the device");
+        HC(hipGetLastError());
         S.t_report_kernel += g_ev_report.seconds();
         S.report_launches += 1;
         u64 oc2[2];
+        int nf = 0;
         HC(hipMemcpy(oc2, ocount.p, sizeof oc2, hipMemcpyDeviceToHost));
-        HC(hipMemcpy(hstatus.data(), status.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
-        HC(hipMemcpy(hedge.data(), edge_out.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
-        HC(hipMemcpy(htouch.data(), touch.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
-        const int li = (int)launches.size();
-        launches.push_back({(long long)oc[0], (long long)oc2[0]});
-        std::vector<int> again;
+        HC(hipMemcpy(&nf, fcount.p, sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int> fl(2 * (size_t)nf);
+        if (nf > 0) HC(hipMemcpy(fl.data(), flist.p, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost));
         bool grow = false;
-        if (all) todo = first_order;
-        for (int j : todo) {
-            const int s = hstatus[j];
-            if (s == RP_OK) final_launch[j] = li;
-            else if (s == RP_UNDEF) {
+        for (int i = 0; i < nf; i++) {
+            const int j = fl[2 * i], st_ = fl[2 * i + 1];
+            if (st_ == RP_UNDEF) {
                 char m[160];
                 snprintf(m, sizeof m, "pair job %d: an alignment reached a wave state the reference leaves undefined", j);
                 throw Fail(m, MI_ERR_UNDEF);
-            } else {
-                again.push_back(j);
-                if (s == RP_OUT) grow = true;
             }
+            if (st_ == RP_OUT) grow = true;
+            again.push_back({j, st_});
         }
+        std::sort(again.begin(), again.end());
         if (grow || (long long)oc2[0] > ocap || (long long)oc2[1] > otcap) {
             // grow the output arenas past what has been reserved so far
             const long long nrec = std::max<long long>(ocap * 2, (long long)oc2[0] * 3 / 2 + 4096);
@@ -878,98 +933,109 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         return again;
     };
 
-    // runs `todo` until every job is done: output-arena overflows are re-run
-    // at the same window, wide bands / full pools at the next one
-    // todo empty: every job, in the device launch order
-    auto run_all = [&](std::vector<int> todo, bool with_edge_in) {
+    // runs a job list until every job is done: output-arena overflows are
+    // re-run at the same window, wide bands / full pools at the next one
+    auto run_all = [&](const int* dorder, long long n) {
         int level = 0;
-        bool first = true;
-        while (first || !todo.empty()) {
-            first = false;
-            std::vector<int> again = run(todo, level, with_edge_in);
+        std::vector<std::pair<int, int>> again = run(dorder, n, level);
+        while (!again.empty()) {
             S.retries += (long long)again.size();
             bool up = false, arena = false;
-            for (int j : again) {
-                up |= hstatus[j] != RP_OUT;
-                arena |= hstatus[j] == RP_ARENA;
+            std::vector<int> todo;
+            for (auto& f : again) {
+                up |= f.second != RP_OUT;
+                arena |= f.second == RP_ARENA;
+                todo.push_back(f.first);
             }
             if (up) {
                 if (level == 3) throw Fail("pairs exceed the largest aligner window / arena");
                 level = arena ? 3 : level + 1;
             }
-            todo.swap(again);
+            HC(hipMemcpyAsync(order.p, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice, st));
+            again = run(order.p, (long long)todo.size(), level);
         }
     };
-    {
-        // exact gate pre-check: pairs whose seeds cannot pass report_thread's
-        // gate are finished here (no alignment, no record); the rest run
-        // longest-first
-        DBuf<u64> keep, kscan, k1, k2;
-        keep.ensure(njobs);
-        kscan.ensure(njobs + 1);
-        k1.ensure(njobs);
-        k2.ensure(njobs);
-        HC(hipMemsetAsync(status.p, 0, sizeof(int) * njobs, st));
-        HC(hipMemsetAsync(edge_out.p, 0, sizeof(int) * njobs, st));
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prefilter), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, (int)njobs,
-                           pstart.p, hits, K, BS, g_hitmin, minhit, mindiag, maxdiag, keep.p, touch.p);
-        HC(hipGetLastError());
-        g_scan.run(keep.p, kscan.p, njobs);
-        nfirst = (long long)d2h1(kscan.p + njobs);
-        S.njobs_kept += nfirst;
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_keys), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, pstart.p, keep.p,
-                           (int)njobs, k1.p);
-        HC(hipGetLastError());
-        u64* sorted = radix_sort(k1.p, k2.p, njobs, 32, 52, KeyU64{});
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_low32), dim3(nblk(njobs)), dim3(256), 0, st, sorted, (int)njobs, order_all.p);
-        HC(hipGetLastError());
-        first_order.resize(nfirst);
-        if (nfirst > 0)
-            HC(hipMemcpyAsync(first_order.data(), order_all.p, sizeof(int) * nfirst, hipMemcpyDeviceToHost, st));
-        HC(hipStreamSynchronize(st));
-        // skipped jobs are complete: status OK, the shared cell cleared when touched
-        HC(hipMemcpy(hstatus.data(), status.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
-        HC(hipMemcpy(htouch.data(), touch.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
-        run_all(std::vector<int>(), false);
-    }
+    run_all(order_all.p, nfirst);
 
     // the cell shared by lastp[maxdiag] and lasta[mindiag-1] carries from
     // one pair to the next within a report chunk: re-run the pairs that
     // would have seen a non-zero value (rare; needs an alignment reaching
-    // the lowest band)
+    // the lowest band).  Only pairs that touch the cell take part: the
+    // others pass it on unchanged.
     {
-        std::vector<int> job_chunk(njobs);
-        for (long long j = 0; j < njobs; j++) {
-            int c = 0;
-            while (c + 1 < nthreads && hcb[c + 1] <= hjstart[j]) c++;
-            job_chunk[j] = c;
-        }
-        std::vector<long long> cstart(nthreads + 1, njobs);
-        for (long long j = njobs - 1; j >= 0; j--) cstart[job_chunk[j]] = j;
-        for (int c = nthreads - 1; c >= 0; c--)
-            if (cstart[c] > cstart[c + 1]) cstart[c] = cstart[c + 1];
-        std::vector<long long> pos(nthreads);
-        std::vector<int> carry(nthreads, 0);
-        for (int c = 0; c < nthreads; c++) pos[c] = cstart[c];
-        for (;;) {
-            std::vector<int> rerun;
-            for (int c = 0; c < nthreads; c++) {
-                long long j = pos[c];
-                const long long end = cstart[c + 1];
-                for (; j < end; j++) {
-                    if (!htouch[j]) continue;
-                    if (hedge_in[j] != carry[c]) {
-                        hedge_in[j] = carry[c];
-                        rerun.push_back((int)j);
-                        break;
-                    }
-                    carry[c] = hedge[j];
-                }
-                pos[c] = j;
+        DBuf<u64> tflag, tpos;
+        tflag.ensure(njobs);
+        tpos.ensure(njobs + 1);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flag_nonzero), dim3(nblk(njobs)), dim3(256), 0, st, touch.p, njobs,
+                           tflag.p);
+        g_scan.run(tflag.p, tpos.p, njobs);
+        const long long nt = (long long)d2h1(tpos.p + njobs);
+        if (nt > 0) {
+            DBuf<int> tj, te;
+            DBuf<long long> ts;
+            tj.ensure(nt);
+            te.ensure(nt);
+            ts.ensure(nt);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact_idx), dim3(nblk(njobs)), dim3(256), 0, st, tflag.p, tpos.p,
+                               njobs, tj.p);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_touched_info), dim3(nblk(nt)), dim3(256), 0, st, tj.p, nt, jobs.p,
+                               pstart.p, edge_out.p, te.p, ts.p);
+            std::vector<int> hj(nt), he(nt);
+            std::vector<long long> hs(nt);
+            HC(hipMemcpyAsync(hj.data(), tj.p, sizeof(int) * nt, hipMemcpyDeviceToHost, st));
+            HC(hipMemcpyAsync(he.data(), te.p, sizeof(int) * nt, hipMemcpyDeviceToHost, st));
+            HC(hipMemcpyAsync(hs.data(), ts.p, sizeof(long long) * nt, hipMemcpyDeviceToHost, st));
+            HC(hipStreamSynchronize(st));
+            std::vector<int> chunk(nt), ein(nt, 0);
+            for (long long i = 0; i < nt; i++) {
+                int c = 0;
+                while (c + 1 < nthreads && hcb[c + 1] <= hs[i]) c++;
+                chunk[i] = c;
             }
-            if (rerun.empty()) break;
-            S.edge_fixups += (long long)rerun.size();
-            run_all(rerun, true);
+            // walk the touched jobs of every chunk in order with the chunk's carry
+            std::vector<long long> pos(nthreads, -1);
+            std::vector<int> carry(nthreads, 0);
+            for (;;) {
+                std::vector<int> rerun;
+                std::vector<long long> ridx;
+                for (long long i = 0; i < nt; i++) {
+                    const int c = chunk[i];
+                    if (i <= pos[c] || pos[c] == -2) continue;
+                    if (ein[i] != carry[c]) {   // first job that saw another value: re-run it
+                        ein[i] = carry[c];
+                        rerun.push_back(hj[i]);
+                        ridx.push_back(i);
+                        pos[c] = -2;   // this chunk waits for the re-run
+                        continue;
+                    }
+                    carry[c] = he[i];
+                    pos[c] = i;
+                }
+                if (rerun.empty()) break;
+                S.edge_fixups += (long long)rerun.size();
+                // new inputs, re-run, and read back the new outputs
+                std::vector<int> vals;
+                for (long long i : ridx) vals.push_back(ein[i]);
+                DBuf<int> rj, rv;
+                rj.ensure(rerun.size());
+                rv.ensure(rerun.size());
+                HC(hipMemcpyAsync(rj.p, rerun.data(), sizeof(int) * rerun.size(), hipMemcpyHostToDevice, st));
+                HC(hipMemcpyAsync(rv.p, vals.data(), sizeof(int) * vals.size(), hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scatter_int), dim3(nblk(rerun.size())), dim3(256), 0, st, rj.p,
+                                   rv.p, (long long)rerun.size(), edge_in.p);
+                HC(hipMemcpyAsync(order.p, rerun.data(), sizeof(int) * rerun.size(), hipMemcpyHostToDevice, st));
+                run_all(order.p, (long long)rerun.size());
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_int), dim3(nblk(rerun.size())), dim3(256), 0, st, rj.p,
+                                   edge_out.p, (long long)rerun.size(), rv.p);
+                HC(hipMemcpy(vals.data(), rv.p, sizeof(int) * vals.size(), hipMemcpyDeviceToHost));
+                for (size_t q = 0; q < ridx.size(); q++) {
+                    const long long i = ridx[q];
+                    he[i] = vals[q];
+                    const int c = chunk[i];
+                    carry[c] = he[i];
+                    pos[c] = i;
+                }
+            }
         }
     }
     const double t2 = now_s();
@@ -981,13 +1047,14 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     // GPU by (aread, bread, abpos, position in the pair's list) -- the order
     // report_thread's buffers give equal (aread, bread, COMP, abpos) keys --
     // compacted with their traces, and copied back in one piece.
-    u64 oc[2], sv[4];
+    u64 oc[2], sv[5];
     HC(hipMemcpy(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost));
     HC(hipMemcpy(sv, stats.p, sizeof sv, hipMemcpyDeviceToHost));
     S.naligned += (long long)sv[0];
     S.cells += (long long)sv[1];
     S.snake += (long long)sv[2];
     S.edge_events += (long long)sv[3];
+    S.wave_steps += (long long)sv[4];
     const long long nrec = std::min<long long>((long long)oc[0], ocap);
     long long nkeep = 0, tbytes_all = 0;
     DBuf<OutRec> srec;
@@ -999,19 +1066,6 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         const int pb = bits_for(std::max(A->maxlen, B->maxlen));
         const int lb = bits_for(std::max(levels[0].mcap, levels[3].mcap));
         if (2 * rb + pb + lb > 64) throw Fail("record keys exceed 64 bits", MI_ERR_ARG);
-        DBuf<int> dfinal;
-        DBuf<long long> dlb, dle;
-        dfinal.ensure(njobs);
-        dlb.ensure(launches.size());
-        dle.ensure(launches.size());
-        std::vector<long long> lbh(launches.size()), leh(launches.size());
-        for (size_t i = 0; i < launches.size(); i++) {
-            lbh[i] = launches[i].first;
-            leh[i] = launches[i].second;
-        }
-        HC(hipMemcpyAsync(dfinal.p, final_launch.data(), sizeof(int) * njobs, hipMemcpyHostToDevice, st));
-        HC(hipMemcpyAsync(dlb.p, lbh.data(), sizeof(long long) * lbh.size(), hipMemcpyHostToDevice, st));
-        HC(hipMemcpyAsync(dle.p, leh.data(), sizeof(long long) * leh.size(), hipMemcpyHostToDevice, st));
         DBuf<RecKey> k1, k2;
         DBuf<u64> flag, pos;
         k1.ensure(nrec);
@@ -1020,7 +1074,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         pos.ensure(nrec + 1);
         const RecKeyFmt kf{rfirst, rb, pb, lb};
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rec_keys), dim3(nblk(nrec)), dim3(256), 0, st, orecs.p, nrec, (int)njobs,
-                           dfinal.p, dlb.p, dle.p, kf, k1.p, flag.p);
+                           final_gen.p, kf, k1.p, flag.p);
         HC(hipGetLastError());
         g_scan.run(flag.p, pos.p, nrec);
         nkeep = (long long)d2h1(pos.p + nrec);

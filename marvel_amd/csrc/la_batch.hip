@@ -62,14 +62,14 @@ __global__ __launch_bounds__(64) void la_batch_kernel(const LaTask* __restrict__
         const LaTask tk = tasks[ti];
         uint16_t* atr = trbuf + tk.tr + tk.maxtp;
         uint16_t* btr = trbuf + tk.tr + 3 * (long long)tk.maxtp;
-        long long cells = 0, snk = 0;
+        WaveCount wc;
         LaResult r = local_alignment<NCH>(sp, pk + tk.apk, tk.alen, pk + tk.bpk, tk.blen, tk.comp, tk.low, tk.hgh,
                                           tk.anti, tk.minp, tk.maxp, pool, pool_cap, pts, pts_cap, atr, btr, tk.maxtp,
-                                          abort_flag, cells, snk);
+                                          abort_flag, wc);
         const int vals[18] = {r.abpos, r.bbpos, r.aepos, r.bepos, r.diffs, r.b_abpos, r.b_bbpos, r.b_aepos,
                               r.b_bepos, r.atlen, r.btlen, r.afirst, r.bfirst, r.status,
-                              (int)(cells & 0xffffffff), (int)(cells >> 32), (int)(snk & 0xffffffff),
-                              (int)(snk >> 32)};
+                              (int)(wc.cells & 0xffffffff), (int)(wc.cells >> 32),
+                              (int)(wc.snake & 0xffffffff), (int)(wc.snake >> 32)};
         int v = 0;
 #pragma unroll
         for (int q = 0; q < 18; q++) v = (lane == q) ? vals[q] : v;

@@ -201,6 +201,11 @@ __device__ __forceinline__ Diag diag_rdl(const Diag& D, int ln)
     return g;
 }
 
+// Work counters of one wave: band cells visited, snake compare steps, wave steps.
+struct WaveCount {
+    long long cells = 0, snake = 0, steps = 0;
+};
+
 struct PassOut {
     int trimx, trimy, trimd;
     int tha, thad, thb, thbd;   // tail pebbles of the A and B chains, and their depths
@@ -225,7 +230,7 @@ __device__ __forceinline__ Diag fetch_diag(const Diag (&bd)[NCH], int base, int 
 template <int NCH, int DIR>
 __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, const uint64_t* B, int blen,
                              int comp, int low, int hgh, int mida, int minp, int maxp, Peb* pool,
-                             int pool_cap, const int* abort_flag, long long& cells, long long& snk)
+                             int pool_cap, const int* abort_flag, WaveCount& wc)
 {
     constexpr int W = 64 * NCH;
     constexpr int SENT = (DIR > 0) ? -1 : INT32_MAX;
@@ -469,7 +474,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             } else
                 D.V = SENT;
         }
-        cells += hgh - low + 1;
+        wc.cells += hgh - low + 1;
+        wc.steps += 1;
 
         // ---- clips (dalign/align.c:589-625, :912-944, :1327-1361, :1650-1686)
         {
@@ -705,7 +711,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
     long long t = my_snake;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
-    snk += t;
+    wc.snake += t;
     return out;
 }
 
@@ -830,7 +836,7 @@ template <int NCH>
 __device__ LaResult local_alignment(const DevSpec& sp, const uint64_t* A, int alen, const uint64_t* B, int blen,
                                     int comp, int low, int hgh, int anti, int minp, int maxp, Peb* pool,
                                     int pool_cap, Peb* pts, int pts_cap, uint16_t* atr, uint16_t* btr, int maxtp,
-                                    const int* abort_flag, long long& cells, long long& snk)
+                                    const int* abort_flag, WaveCount& wc)
 {
     LaResult r;
     const int lane = __lane_id();
@@ -839,7 +845,7 @@ __device__ LaResult local_alignment(const DevSpec& sp, const uint64_t* A, int al
     r.status = LA_OK;
 
     PassOut f = wave_pass<NCH, +1>(sp, A, alen, B, blen, comp, low, hgh, anti, minp, maxp, pool, pool_cap, abort_flag,
-                                   cells, snk);
+                                   wc);
     if (f.status != LA_OK) { r.status = f.status; return r; }
     // each trace holds at most maxtp values on either side of its origin
     if (max(f.thad, f.thbd) + 1 > pts_cap || 2 * max(f.thad, f.thbd) + 2 > maxtp) { r.status = LA_CHAIN; return r; }
@@ -863,7 +869,7 @@ __device__ LaResult local_alignment(const DevSpec& sp, const uint64_t* A, int al
     wave_sync();   // the pool is reused by the reverse pass
 
     PassOut v = wave_pass<NCH, -1>(sp, A, alen, B, blen, comp, mind, mind, anti, minp, maxp, pool, pool_cap,
-                                   abort_flag, cells, snk);
+                                   abort_flag, wc);
     if (v.status != LA_OK) { r.status = v.status; return r; }
     if (max(v.thad, v.thbd) + 1 > pts_cap || 2 * max(v.thad, v.thbd) + 2 > maxtp) { r.status = LA_CHAIN; return r; }
     wave_sync();

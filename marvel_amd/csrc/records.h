@@ -11,6 +11,8 @@ struct OutRec {   // one .las record plus its sort tie-breaker
     int local;        // position in the pair's kept list (ties of the final sort)
     int job;          // job that wrote the record (-1: slot never written)
     long long toff;   // offset of the trace bytes in the output trace arena
+    int gen;          // launch that wrote it (a job's records count from its last successful launch)
+    int pad;
 };
 
 }  // namespace mi

@@ -82,9 +82,11 @@ struct ReportArgs {
     long long otcap;
     unsigned long long* ocount;   // [0] records, [1] trace bytes
     int* status;                  // per job
+    int* final_gen;               // per job: launch of its last successful run
+    int gen;                      // this launch
     int* counter;
     const int* abort_flag;
-    unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events
+    unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events, [4] wave steps
 };
 
 // ---- redundancy handling (dalign/filter.c:1249-1566), lane 0 ------------------
@@ -266,7 +268,8 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
         int pstatus = RP_OK;
         int tmin = INT32_MAX, tmax = -INT32_MAX;   // lasta cells raised by this pair
         bool touched = false;
-        long long aligned = 0, cells = 0, snk = 0, edges = 0;
+        long long aligned = 0, edges = 0;
+        WaveCount wc;
         {
             const int e0 = a.edge_in ? a.edge_in[job] : 0;
             if (lane == 0) *edge = e0;
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
                     const int ap = sd.apos, dg = sd.diag, bp = ap - dg, band0 = dg >> a.binshift;
                     LaResult r = local_alignment<NCH>(a.sp, Aseq, alen, Bseq, blen, a.comp, dg, dg, ap + bp,
                                                       -INT32_MAX, INT32_MAX, pool, a.pool_cap, pts, a.pts_cap, atr,
-                                                      btr, maxtp, a.abort_flag, cells, snk);
+                                                      btr, maxtp, a.abort_flag, wc);
                     if (r.status != LA_OK) {
                         pstatus = (r.status == LA_WIDE || r.status == LA_POOL || r.status == LA_ABORT) ? RP_RETRY
                                                                                                         : RP_UNDEF;
@@ -485,6 +488,8 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
                             o.local = isA ? i : i - na;
                             o.job = job;
                             o.toff = (long long)to;
+                            o.gen = a.gen;
+                            o.pad = 0;
                             a.orecs[rbase + i] = o;
                         }
                         // Compress_TraceTo8 (dalign/align.c:3352-3360) when the spacing is <= 125
@@ -505,13 +510,15 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
         const int eout = *edge;
         if (lane == 0) {
             a.status[job] = pstatus;
+            if (pstatus == RP_OK) a.final_gen[job] = a.gen;
             a.edge_out[job] = eout;
             a.touch[job] = touched ? 1 : 0;
             if (pstatus == RP_OK) {
                 atomicAdd(&a.stats[0], (unsigned long long)aligned);
-                atomicAdd(&a.stats[1], (unsigned long long)cells);
-                atomicAdd(&a.stats[2], (unsigned long long)snk);
+                atomicAdd(&a.stats[1], (unsigned long long)wc.cells);
+                atomicAdd(&a.stats[2], (unsigned long long)wc.snake);
                 atomicAdd(&a.stats[3], (unsigned long long)edges);
+                atomicAdd(&a.stats[4], (unsigned long long)wc.steps);
             }
         }
         // leave the band arrays zero for the next job: the edge cell and the

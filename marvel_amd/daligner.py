@@ -45,7 +45,8 @@ class MiFilterStats(C.Structure):
                 ("edge_events", C.c_longlong), ("edge_fixups", C.c_longlong), ("retries", C.c_longlong),
                 ("limit", C.c_int), ("t_filter", C.c_double), ("t_report", C.c_double), ("t_copy", C.c_double),
                 ("t_total", C.c_double), ("t_report_kernel", C.c_double), ("report_launches", C.c_longlong),
-                ("t_seed_sort", C.c_double), ("seed_sort_passes", C.c_longlong), ("njobs_kept", C.c_longlong)]
+                ("t_seed_sort", C.c_double), ("seed_sort_passes", C.c_longlong), ("njobs_kept", C.c_longlong),
+                ("wave_steps", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

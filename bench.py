@@ -60,6 +60,65 @@ def las_summary(path):
     return n, bases, hashlib.sha256(raw).hexdigest()
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE in
+    KiB, scaled by the factors a plain 1 GiB copy gives).  None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_report_pmc_5mbp.md")
+    if not os.path.exists(path):
+        return None
+    fetch_scale = write_scale = None
+    row = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and cells[0] == "FETCH_SIZE (KiB)":
+            fetch_scale = float(cells[2].split()[0])
+        elif len(cells) >= 3 and cells[0] == "WRITE_SIZE (KiB)":
+            write_scale = float(cells[2].split()[0])
+        elif len(cells) >= 3 and cells[0] == f"`{kernel}`" and row is None:
+            row = cells
+    if row is None or fetch_scale is None or write_scale is None:
+        return None
+    kib = lambda c: float(c.replace(",", ""))
+    return int((kib(row[1]) * fetch_scale + kib(row[2]) * write_scale) * 1024)
+
+
+def gated_leg(dal, A, spec, dev, cA, d, exact_las, steps):
+    """The same step with the learned candidate scorer gating the aligner
+    (shipped weights, trained on this workload's captures and a repeat-rich
+    genome): overlaps/s and the fraction of the exact run's records it keeps."""
+    from marvel_amd import scorer as S
+    if not os.path.exists(S.DEFAULT_WEIGHTS):
+        return None
+    W = S.ScorerWeights.load()
+    sc = S.GpuScorer(dal.h)
+    sc.load(W)
+    ref = os.path.join(d, "exact.las")
+    shutil.copyfile(exact_las, ref)
+    times, stats = [], None
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        sc.mode(2)
+        for _ in range(steps):
+            dal.synchronize()
+            t = time.perf_counter()
+            stats = dal.compare_self(A, spec, 1, "C2.1", dev=dev, cA=cA)
+            dal.synchronize()
+            times.append(time.perf_counter() - t)
+    finally:
+        sc.mode(0)
+        os.chdir(cwd)
+    n, bases, _ = las_summary(exact_las)
+    rec, hit, tot = S.record_recall(ref, exact_las)
+    step = sum(times) / len(times)
+    return {"value": round(n / step, 1), "unit": "overlaps/s", "ms_per_step": round(step * 1e3, 2),
+            "record_recall": round(rec, 6), "records": n, "records_of_exact_kept": hit, "records_exact": tot,
+            "candidates_scored": sum(x["cand_scored"] for x in stats),
+            "candidates_skipped": sum(x["cand_skipped"] for x in stats),
+            "kernel_ms_per_step": round(sum(x["t_report_kernel"] for x in stats) * 1e3, 2),
+            "threshold": W.threshold, "trained_on": W.meta.get("configs")}
+
+
 def workdir():
     base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else None
     return tempfile.mkdtemp(prefix="mibench_", dir=base)
@@ -102,6 +161,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-gated", action="store_true", help="skip the scorer-gated leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -171,6 +231,7 @@ def main():
         t_sort = sum(a["t_seed_sort"] + b["t_seed_sort"] for a, b in stats) / len(stats)
         sort_bytes = sum(x["nhits"] * 32 * x["seed_sort_passes"] for x in stats[-1])
         sort_gbs = sort_bytes / t_sort / 1e9 if t_sort > 0 else 0.0
+        gated = None if args.no_gated else gated_leg(dal, A, spec, dev, cA, d, las, args.steps)
         cpu = None if args.no_cpu else cpu_baseline(reads, d)
         line = {
             "metric": METRIC,
@@ -193,7 +254,8 @@ def main():
             "parity": "bit-exact vs reference daligner" if sha == GOLD_SHA else f"MISMATCH sha256 {sha}",
             "roofline": {"bound": "valu", "kernel": "k_report (one wave64 per read pair: gating + Local_Alignment)",
                          "achieved": round(ach, 2), "peak": VALU_PEAK_GOPS, "unit": "Gop/s (W_align lane-ops)",
-                         "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": None,
+                         "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": profiled_traffic("k_report<1>"),
+                         "traffic_unit": "HBM bytes per k_report<1> launch (rocprofv3 --pmc, calibrated)",
                          "kernel_ms_per_step": round(k_report * 1e3, 2), "launches_per_step": launches,
                          "w_align_per_step": w_align},
             "roofline_hbm": {"bound": "hbm", "kernel": "seed LSD radix sort (rs_hist + rs_scatter)",
@@ -201,6 +263,7 @@ def main():
                              "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                              "ms_per_step": round(t_sort * 1e3, 2)},
             "cpu_baseline": cpu,
+            "scorer_gated": gated,
             "times_s": [round(x, 4) for x in times],
             "phases_s": phases[-1],
             "setup_s": {"db_write": round(t_db, 2), "load_and_upload": round(t_load, 2)},

@@ -103,12 +103,32 @@ typedef struct {
     long long seed_sort_passes;     /* 8-bit digit passes of that sort */
     long long njobs_kept;           /* pairs past the exact gate pre-check (the report kernel's jobs) */
     long long wave_steps;           /* aligner wave steps (one per difference level and pass) */
+    long long cand_scored;          /* candidates the scorer evaluated (gating mode) */
+    long long cand_skipped;         /* candidates it kept from being aligned */
 } MiFilterStats;
 
 /* Match_Filter returning an error code (MI_ERR_*) instead of exiting. */
 int mi_match_filter(char *aname, HITS_DB *ablock, char *bname, HITS_DB *bblock, void *atable, void *btable,
                     int comp, Align_Spec *asettings);
 void mi_filter_stats(MiFilterStats *out);
+
+/* ---- candidate scorer (no reference counterpart) ---------------------------
+ * An MLP 16 -> 64 -> 64 -> 1 (ReLU; bf16 weights and activations, fp32
+ * accumulation, gfx950 MFMA) over 16 features of each alignment candidate of
+ * report_thread (a seed that passes the band gate, dalign/filter.c:1772-1778).
+ * Weights are input-major: w1[16*64], w2[64*64], w3[64].  Mode 0 aligns every
+ * candidate (bit-exact with the reference), 1 does the same and records each
+ * aligned candidate's features with label 1 when its alignment was kept
+ * ((aepos-abpos)+(bepos-bbpos) >= 2*minlen), 2 skips candidates whose logit
+ * is below the threshold. */
+int mi_scorer_load(const float *w1, const float *b1, const float *w2, const float *b2, const float *w3, float b3,
+                   float threshold);
+int mi_scorer_mode(int mode);
+long long mi_scorer_captured(void);
+/* copies up to cap captured rows (16 floats + a label each) and clears the store */
+long long mi_scorer_take(float *feat, int *label, long long cap);
+/* logits of n feature rows with the loaded weights, on the GPU */
+int mi_mlp_score(const float *feat, long long n, float *logits);
 
 /* Index of the reverse complement of the block indexed by `table`, built on
  * the GPU from the packed reads the handle holds (the complement the

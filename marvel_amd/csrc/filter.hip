@@ -592,10 +592,24 @@ __global__ __launch_bounds__(64) void k_trace_gather(OutRec* __restrict__ recs, 
     if (threadIdx.x == 0) recs[i].toff = (long long)pos[i];
 }
 
+// candidate scorer state (scorer.hip)
+int scorer_mode();
+bool scorer_weights(MlpW* w, float* thr);
+void scorer_capture_append(const float* feat, const int* label, long long n);
+
 template <int NCH>
-static void launch_report(const ReportArgs& a, int grid)
+static void launch_report(const ReportArgs& a, int grid, int sc)
 {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH>), dim3(grid), dim3(64), 0, fstream(), a);
+    switch (sc) {
+    case SC_CAPTURE:
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, SC_CAPTURE>), dim3(grid), dim3(64), 0, fstream(), a);
+        break;
+    case SC_GATE:
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, SC_GATE>), dim3(grid), dim3(64), 0, fstream(), a);
+        break;
+    default:
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, SC_EXACT>), dim3(grid), dim3(64), 0, fstream(), a);
+    }
 }
 
 static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, SpecObj* spec)
@@ -751,9 +765,9 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     counter.ensure(1);
     fcount.ensure(1);
     ocount.ensure(2);
-    stats.ensure(5);
+    stats.ensure(7);
     HC(hipMemsetAsync(ocount.p, 0, 2 * sizeof(u64), st));
-    HC(hipMemsetAsync(stats.p, 0, 5 * sizeof(u64), st));
+    HC(hipMemsetAsync(stats.p, 0, 7 * sizeof(u64), st));
     HC(hipMemsetAsync(edge_in.p, 0, sizeof(int) * njobs, st));
     HC(hipMemsetAsync(edge_out.p, 0, sizeof(int) * njobs, st));
     HC(hipMemsetAsync(status.p, 0, sizeof(int) * njobs, st));
@@ -831,6 +845,23 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     a.abort_flag = mi_abort_flag_device();
     a.stats = stats.p;
     a.ocount = ocount.p;
+    // candidate scorer: gate with the loaded weights, or capture features + outcomes
+    const int sc_mode = scorer_mode();
+    if (sc_mode == SC_GATE && !scorer_weights(&a.mlp, &a.mlp_thr))
+        throw Fail("candidate gating requested without scorer weights", MI_ERR_ARG);
+    DBuf<float> capf;
+    DBuf<int> capm;
+    DBuf<u64> capn;
+    if (sc_mode == SC_CAPTURE) {
+        a.cap_cap = std::max<long long>(1 << 16, 8 * nfirst);
+        capf.ensure((size_t)a.cap_cap * kFeat);
+        capm.ensure((size_t)a.cap_cap * 4);
+        capn.ensure(1);
+        HC(hipMemsetAsync(capn.p, 0, sizeof(u64), st));
+        a.cap_feat = capf.p;
+        a.cap_meta = capm.p;
+        a.cap_count = capn.p;
+    }
 
     int cur_level = -1, gen = 0;
 
@@ -880,9 +911,9 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         b.otcap = otcap;
         g_ev_report.record_start();
         switch (L.nch) {
-        case 1: launch_report<1>(b, grid); break;
-        case 4: launch_report<4>(b, grid); break;
-        default: launch_report<16>(b, grid); break;
+        case 1: launch_report<1>(b, grid, sc_mode); break;
+        case 4: launch_report<4>(b, grid, sc_mode); break;
+        default: launch_report<16>(b, grid, sc_mode); break;
         }
         HC(hipGetLastError());
         g_ev_report.record_stop();
@@ -1047,7 +1078,26 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     // GPU by (aread, bread, abpos, position in the pair's list) -- the order
     // report_thread's buffers give equal (aread, bread, COMP, abpos) keys --
     // compacted with their traces, and copied back in one piece.
-    u64 oc[2], sv[5];
+    if (sc_mode == SC_CAPTURE) {
+        // keep the rows of every job's final launch
+        const long long nc = std::min<long long>((long long)d2h1(capn.p), a.cap_cap);
+        std::vector<float> hf((size_t)nc * kFeat);
+        std::vector<int> hm((size_t)nc * 4), hg(njobs);
+        if (nc > 0) {
+            HC(hipMemcpy(hf.data(), capf.p, sizeof(float) * hf.size(), hipMemcpyDeviceToHost));
+            HC(hipMemcpy(hm.data(), capm.p, sizeof(int) * hm.size(), hipMemcpyDeviceToHost));
+            HC(hipMemcpy(hg.data(), final_gen.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
+        }
+        std::vector<float> kf;
+        std::vector<int> kl;
+        for (long long i = 0; i < nc; i++) {
+            if (hg[hm[4 * i + 1]] != hm[4 * i + 2]) continue;
+            kf.insert(kf.end(), hf.begin() + i * kFeat, hf.begin() + (i + 1) * kFeat);
+            kl.push_back(hm[4 * i]);
+        }
+        scorer_capture_append(kf.data(), kl.data(), (long long)kl.size());
+    }
+    u64 oc[2], sv[7];
     HC(hipMemcpy(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost));
     HC(hipMemcpy(sv, stats.p, sizeof sv, hipMemcpyDeviceToHost));
     S.naligned += (long long)sv[0];
@@ -1055,6 +1105,8 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     S.snake += (long long)sv[2];
     S.edge_events += (long long)sv[3];
     S.wave_steps += (long long)sv[4];
+    S.cand_scored += (long long)sv[5];
+    S.cand_skipped += (long long)sv[6];
     const long long nrec = std::min<long long>((long long)oc[0], ocap);
     long long nkeep = 0, tbytes_all = 0;
     DBuf<OutRec> srec;

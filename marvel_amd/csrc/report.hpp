@@ -26,6 +26,7 @@
 #include "filter_kernels.hpp"
 #include "la_wave.hpp"
 #include "records.h"
+#include "scorer.hpp"
 
 namespace mi {
 
@@ -86,8 +87,21 @@ struct ReportArgs {
     int gen;                      // this launch
     int* counter;
     const int* abort_flag;
-    unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events, [4] wave steps
+    unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events, [4] wave steps,
+                                  // [5] candidates scored, [6] candidates skipped by the scorer
+    // candidate scorer (report kernel instantiations with SC != 0)
+    MlpW mlp;                     // SC == 2: weights
+    float mlp_thr;                //          align when logit >= mlp_thr
+    float* cap_feat;              // SC == 1: kFeat features per captured candidate
+    int* cap_meta;                //          4 ints: kept, job, launch, 0
+    unsigned long long* cap_count;
+    long long cap_cap;
 };
+
+// report kernel variants
+constexpr int SC_EXACT = 0;     // every candidate aligned (the reference's behaviour)
+constexpr int SC_CAPTURE = 1;   // exact, and every aligned candidate's features + outcome recorded
+constexpr int SC_GATE = 2;      // candidates whose scorer logit is below the threshold are not aligned
 
 // ---- redundancy handling (dalign/filter.c:1249-1566), lane 0 ------------------
 
@@ -227,7 +241,7 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
 
 // ---- the pair kernel --------------------------------------------------------------
 
-template <int NCH>
+template <int NCH, int SC>
 __global__ __launch_bounds__(64) void k_report(ReportArgs a)
 {
     const int lane = __lane_id();
@@ -268,7 +282,8 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
         int pstatus = RP_OK;
         int tmin = INT32_MAX, tmax = -INT32_MAX;   // lasta cells raised by this pair
         bool touched = false;
-        long long aligned = 0, edges = 0;
+        long long aligned = 0, edges = 0, scored = 0, skipped = 0;
+        int kept = 0;
         WaveCount wc;
         {
             const int e0 = a.edge_in ? a.edge_in[job] : 0;
@@ -354,6 +369,29 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
                     f0 = fh + 1;
                     const Seed sd = a.hits[fh];
                     const int ap = sd.apos, dg = sd.diag, bp = ap - dg, band0 = dg >> a.binshift;
+                    float feats[kFeat];
+                    if constexpr (SC != SC_EXACT) {
+                        const CandIn ci{score[band0], score[band0 - 1], score[band0 + 1], a.hitmin, pe - ps, fh - ps,
+                                        nidx - lidx, ap, bp, dg, alen, blen, a.comp, (int)aligned, kept,
+                                        lasta[band0]};
+                        cand_features(ci, feats);
+                    }
+                    if constexpr (SC == SC_GATE) {
+                        __shared__ __attribute__((aligned(16))) uint16_t mlp_lds[2 * kMlpLdsHalves];
+                        // weight pointers made opaque here so that their loads stay
+                        // inside this block instead of being hoisted out of the seed
+                        // loop (they would hold ~56 VGPRs across every alignment)
+                        MlpW wl = a.mlp;
+                        asm volatile("" : "+v"(wl.w1), "+v"(wl.w2), "+v"(wl.w3), "+v"(wl.b1), "+v"(wl.b2));
+                        const f32x4 o = mlp_tile(cand_operand(feats), wl, mlp_lds);
+                        const float logit = __builtin_bit_cast(
+                            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, o[0]), 0)) + a.mlp.b3;
+                        scored++;
+                        if (logit < a.mlp_thr) {   // predicted not to yield a kept overlap
+                            skipped++;
+                            continue;
+                        }
+                    }
                     LaResult r = local_alignment<NCH>(a.sp, Aseq, alen, Bseq, blen, a.comp, dg, dg, ap + bp,
                                                       -INT32_MAX, INT32_MAX, pool, a.pool_cap, pts, a.pts_cap, atr,
                                                       btr, maxtp, a.abort_flag, wc);
@@ -395,7 +433,22 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
                     for (int i = lane; i <= hi - lo; i += 64)
                         if (r.aepos > lasta[lo + i]) lasta[lo + i] = r.aepos;
                     wave_sync();
+                    if constexpr (SC == SC_CAPTURE) {
+                        const bool kp = (r.aepos - r.abpos) + (r.bepos - r.bbpos) >= a.minover;
+                        if (lane == 0) {
+                            const unsigned long long q = atomicAdd(a.cap_count, 1ull);
+                            if ((long long)q < a.cap_cap) {
+#pragma unroll
+                                for (int i = 0; i < kFeat; i++) a.cap_feat[q * kFeat + i] = feats[i];
+                                a.cap_meta[4 * q] = kp ? 1 : 0;
+                                a.cap_meta[4 * q + 1] = job;
+                                a.cap_meta[4 * q + 2] = a.gen;
+                                a.cap_meta[4 * q + 3] = 0;
+                            }
+                        }
+                    }
                     if ((r.aepos - r.abpos) + (r.bepos - r.bbpos) >= a.minover) {
+                        kept++;
                         const long long need = (doA ? r.atlen : 0) + (doB ? r.btlen : 0);
                         if (novla >= a.mcap || novlb >= a.mcap || ttop + need > a.mtcap) {
                             pstatus = RP_ARENA;
@@ -519,6 +572,10 @@ __global__ __launch_bounds__(64) void k_report(ReportArgs a)
                 atomicAdd(&a.stats[2], (unsigned long long)wc.snake);
                 atomicAdd(&a.stats[3], (unsigned long long)edges);
                 atomicAdd(&a.stats[4], (unsigned long long)wc.steps);
+                if (SC == SC_GATE) {
+                    atomicAdd(&a.stats[5], (unsigned long long)scored);
+                    atomicAdd(&a.stats[6], (unsigned long long)skipped);
+                }
             }
         }
         // leave the band arrays zero for the next job: the edge cell and the

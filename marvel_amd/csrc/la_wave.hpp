@@ -128,6 +128,43 @@ __device__ __forceinline__ uint64_t lane_above64(uint64_t v)
 {
     return ((uint64_t)(uint32_t)lane_above((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)lane_above((int)(uint32_t)v);
 }
+// Wave-wide scans from DPP row moves (no LDS): row_shl:n / row_shr:n read
+// lane L+n / L-n of the same 16-lane row (lanes past the row keep the
+// identity), row_bcast:15 / :31 hand the last lane of a row (of two rows)
+// to the next row(s).
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_or(int ident, int v)
+{
+    return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROW_MASK, 0xF, false);
+}
+// inclusive max over lanes >= L
+__device__ __forceinline__ int wave_suffix_max(int x)
+{
+    constexpr int ID = -2147483647 - 1;
+    x = max(x, dpp_or<0x101>(ID, x));   // row_shl:1
+    x = max(x, dpp_or<0x102>(ID, x));   // row_shl:2
+    x = max(x, dpp_or<0x104>(ID, x));   // row_shl:4
+    x = max(x, dpp_or<0x108>(ID, x));   // row_shl:8
+    // lane 16r now holds row r's maximum
+    const int s1 = __builtin_amdgcn_readlane(x, 16), s2 = __builtin_amdgcn_readlane(x, 32);
+    const int s3 = __builtin_amdgcn_readlane(x, 48);
+    const int t2 = max(s2, s3), t1 = max(s1, t2);
+    const int row = __lane_id() >> 4;
+    return max(x, row == 0 ? t1 : (row == 1 ? t2 : (row == 2 ? s3 : ID)));
+}
+// inclusive min over lanes <= L
+__device__ __forceinline__ int wave_prefix_min(int x)
+{
+    constexpr int ID = 2147483647;
+    x = min(x, dpp_or<0x111>(ID, x));        // row_shr:1
+    x = min(x, dpp_or<0x112>(ID, x));        // row_shr:2
+    x = min(x, dpp_or<0x114>(ID, x));        // row_shr:4
+    x = min(x, dpp_or<0x118>(ID, x));        // row_shr:8
+    x = min(x, dpp_or<0x142, 0xA>(ID, x));   // row_bcast:15 into rows 1, 3
+    x = min(x, dpp_or<0x143, 0xC>(ID, x));   // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 __device__ __forceinline__ uint64_t rdl64(uint64_t v, int lane)
 {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
@@ -255,7 +292,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
     int morea = mida, morey = besty, mored = 0, moreha = 0, morehad = 0, morehb = 1, morehbd = 0;
     int morem = -1, lasta = mida, more = 1;
     int dif = 0;
-    long long my_snake = 0;
+    int my_snake = 0;   // this lane's snake compare steps in the pass
 
     int cN[NCH], yN[NCH], mN[NCH];
     uint64_t bN[NCH];
@@ -391,21 +428,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             for (int jj = 0; jj < NCH; jj++) {
                 const int j = (DIR > 0) ? (NCH - 1 - jj) : jj;
                 const int c = cN[j];
-                int incl = c;
-                if (DIR > 0) {
-#pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int v = __shfl_down(incl, o, 64);
-                        if (lane + o < 64) incl = max(incl, v);
-                    }
-                } else {
-#pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int v = __shfl_up(incl, o, 64);
-                        if (lane - o >= 0) incl = min(incl, v);
-                    }
-                }
-                int excl = (DIR > 0) ? __shfl_down(incl, 1, 64) : __shfl_up(incl, 1, 64);
+                const int incl = (DIR > 0) ? wave_suffix_max(c) : wave_prefix_min(c);
+                int excl = (DIR > 0) ? lane_above(incl) : lane_below(incl);
                 if (DIR > 0 && lane == 63) excl = SENT;
                 if (DIR < 0 && lane == 0) excl = SENT;
                 const int prior = (DIR > 0) ? max(excl, carry) : min(excl, carry);
@@ -463,16 +487,18 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         }
 
         // ---- commit the wave ------------------------------------------------
+        // Only V needs the band test: a lane outside the band holds SENT, and
+        // an in-band V is an antidiagonal >= 0 (forward) / < SENT (reverse),
+        // so the predecessor choice never takes an out-of-band lane's other
+        // fields.
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
             const int k = base + 64 * j + lane;
             Diag& D = bd[j];
-            if (k >= low && k <= hgh) {
-                D.V = cN[j]; D.M = mN[j]; D.T = bN[j];
-                D.HA = P[j].HA; D.HAm = P[j].HAm; D.HAd = P[j].HAd;
-                D.HB = P[j].HB; D.HBm = P[j].HBm; D.HBd = P[j].HBd;
-            } else
-                D.V = SENT;
+            D.V = (k >= low && k <= hgh) ? cN[j] : SENT;
+            D.M = mN[j]; D.T = bN[j];
+            D.HA = P[j].HA; D.HAm = P[j].HAm; D.HAd = P[j].HAd;
+            D.HB = P[j].HB; D.HBm = P[j].HBm; D.HBd = P[j].HBd;
         }
         wc.cells += hgh - low + 1;
         wc.steps += 1;

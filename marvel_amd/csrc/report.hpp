@@ -242,7 +242,7 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
 // ---- the pair kernel --------------------------------------------------------------
 
 template <int NCH, int SC>
-__global__ __launch_bounds__(64) void k_report(ReportArgs a)
+__global__ __launch_bounds__(64, NCH == 1 ? 4 : 1) void k_report(ReportArgs a)
 {
     const int lane = __lane_id();
     const int wave = blockIdx.x;

@@ -71,8 +71,10 @@ __global__ __launch_bounds__(64) void st_dpp(const int* in, int* out)
 {
     const int lane = __lane_id();
     const int v = in[blockIdx.x * 64 + lane];
-    out[(blockIdx.x * 64 + lane) * 2] = lane_below(v);
-    out[(blockIdx.x * 64 + lane) * 2 + 1] = lane_above(v);
+    out[(blockIdx.x * 64 + lane) * 4] = lane_below(v);
+    out[(blockIdx.x * 64 + lane) * 4 + 1] = lane_above(v);
+    out[(blockIdx.x * 64 + lane) * 4 + 2] = wave_suffix_max(v);
+    out[(blockIdx.x * 64 + lane) * 4 + 3] = wave_prefix_min(v);
 }
 
 }  // namespace mi

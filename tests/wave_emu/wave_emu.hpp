@@ -86,17 +86,28 @@ inline unsigned long long __ballot(int p)
 }
 inline int __any(int p) { return __ballot(p) != 0; }
 inline int __builtin_amdgcn_readlane(int v, int l) { return emu::xchg(v, l); }
-// DPP wave moves: 0x13C wave_ror:1, 0x134 wave_rol:1, 0x138 wave_shr:1, 0x130 wave_shl:1
-inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, bool)
+// DPP moves: 0x13C wave_ror:1, 0x134 wave_rol:1, 0x138 wave_shr:1, 0x130 wave_shl:1,
+// 0x101-0x10F row_shl:n, 0x111-0x11F row_shr:n, 0x142 row_bcast:15, 0x143 row_bcast:31
+// (bank_mask 0xF and bound_ctrl false only: lanes without a source keep `old`)
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int, bool)
 {
-    const int l = emu::lane;
+    const int l = emu::lane, row = l >> 4, rl = l & 15;
+    int from = -1;
     switch (ctrl) {
-    case 0x13C: return emu::xchg(src, (l + 63) & 63);
-    case 0x134: return emu::xchg(src, (l + 1) & 63);
-    case 0x138: { const int v = emu::xchg(src, (l + 63) & 63); return l == 0 ? old : v; }
-    case 0x130: { const int v = emu::xchg(src, (l + 1) & 63); return l == 63 ? old : v; }
-    default: __builtin_trap();
+    case 0x13C: from = (l + 63) & 63; break;
+    case 0x134: from = (l + 1) & 63; break;
+    case 0x138: from = l == 0 ? -1 : l - 1; break;
+    case 0x130: from = l == 63 ? -1 : l + 1; break;
+    case 0x142: from = row == 0 ? -1 : 16 * row - 1; break;
+    case 0x143: from = row < 2 ? -1 : 31; break;
+    default:
+        if (ctrl >= 0x101 && ctrl <= 0x10F) from = rl + (ctrl - 0x100) < 16 ? l + (ctrl - 0x100) : -1;
+        else if (ctrl >= 0x111 && ctrl <= 0x11F) from = rl - (ctrl - 0x110) >= 0 ? l - (ctrl - 0x110) : -1;
+        else __builtin_trap();
     }
+    const int v = emu::xchg(src, from < 0 ? l : from);   // every lane takes part in the exchange
+    if (!((row_mask >> row) & 1) || from < 0) return old;
+    return v;
 }
 inline int __builtin_amdgcn_readfirstlane(int v) { return emu::xchg(v, 0); }
 inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned acc)

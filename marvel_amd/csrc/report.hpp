@@ -241,8 +241,12 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
 
 // ---- the pair kernel --------------------------------------------------------------
 
+// The 64-lane window (NCH = 1) runs at 8 waves per SIMD (64 VGPRs, a few
+// spills): the aligner is bound by VALU issue and dependent latencies, and
+// measured on the 5 Mbp / 30x workload 3 -> 4 -> 5 -> 6 -> 8 waves per SIMD
+// took the kernel from 380 to 305, 277, 258 and 248 ms per step.
 template <int NCH, int SC>
-__global__ __launch_bounds__(64, NCH == 1 ? 4 : 1) void k_report(ReportArgs a)
+__global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
 {
     const int lane = __lane_id();
     const int wave = blockIdx.x;

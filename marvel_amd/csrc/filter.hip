@@ -106,6 +106,26 @@ struct DevCache {
 };
 static DevCache g_cache;
 
+// Grow-only pinned host staging buffer (records and traces coming back from
+// the device); kept for the life of the process.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void* get(size_t need)
+    {
+        if (need > n) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            n = 0;
+            const size_t want = need + need / 4;
+            if (hipHostMalloc(&p, want) != hipSuccess) throw Fail("pinned host allocation failed");
+            n = want;
+        }
+        return p;
+    }
+};
+static PinnedBuf g_pin_rec, g_pin_tr;
+
 template <class T>
 struct DBuf {
     T* p = nullptr;
@@ -1146,13 +1166,13 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
                            otr.p, tpos.p, nkeep, tbytes, str.p);
         HC(hipGetLastError());
     }
-    std::vector<OutRec> recs(nkeep);
-    std::vector<uint8_t> tr(tbytes_all);
-    if (nkeep > 0) HC(hipMemcpyAsync(recs.data(), srec.p, sizeof(OutRec) * nkeep, hipMemcpyDeviceToHost, st));
-    if (tbytes_all > 0) HC(hipMemcpyAsync(tr.data(), str.p, tbytes_all, hipMemcpyDeviceToHost, st));
+    OutRec* recs = (OutRec*)g_pin_rec.get(sizeof(OutRec) * std::max(nkeep, 1LL));
+    uint8_t* tr = (uint8_t*)g_pin_tr.get(std::max(tbytes_all, 1LL));
+    if (nkeep > 0) HC(hipMemcpyAsync(recs, srec.p, sizeof(OutRec) * nkeep, hipMemcpyDeviceToHost, st));
+    if (tbytes_all > 0) HC(hipMemcpyAsync(tr, str.p, tbytes_all, hipMemcpyDeviceToHost, st));
     HC(hipStreamSynchronize(st));
     // srec[i].toff now holds the offset in tr
-    if (mi_append_records(spec->iobuf, recs.data(), nkeep, tr.data(), tbytes_all, tbytes) != 0)
+    if (mi_append_records(spec->iobuf, recs, nkeep, tr, tbytes_all, tbytes) != 0)
         throw Fail("cannot grow the overlap buffer");
     S.nrecords += nkeep;
     S.t_copy += now_s() - t2;

@@ -40,7 +40,7 @@ static_assert(sizeof(LaOut) == 64 * sizeof(int), "output record is one int per l
 constexpr int kOutInts = 64;
 
 template <int NCH>
-__global__ __launch_bounds__(64) void la_batch_kernel(const LaTask* __restrict__ tasks, const int* __restrict__ order,
+__global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void la_batch_kernel(const LaTask* __restrict__ tasks, const int* __restrict__ order,
                                                       int ntasks, const uint64_t* __restrict__ pk, DevSpec sp,
                                                       uint16_t* __restrict__ trbuf, int* __restrict__ outs,
                                                       Peb* __restrict__ pools, int pool_cap, Peb* __restrict__ ptsbuf,
@@ -170,8 +170,10 @@ int mi_la_batch_host(const MiAlignParams* prm, const uint64_t* packed, long long
         int level = 0;
         while (!todo.empty()) {
             const int nch = nchs[std::min(level, 2)];
-            const int pc = (level >= 3) ? pool_cap << 3 : pool_cap;
-            int g = std::min<int>(grid, (int)todo.size());
+            // level 0 (64-lane window) runs 8 waves per SIMD with smaller pools;
+            // tasks that fill them re-run at the next level
+            const int pc = (level == 0) ? pool_cap >> 2 : ((level >= 3) ? pool_cap << 3 : pool_cap);
+            int g = std::min<int>(level == 0 ? grid * 4 : grid, (int)todo.size());
             if (level >= 3) g = std::min(g, 256);
             if (dpool) { (void)hipFree(dpool); dpool = nullptr; }
             if (dpts) { (void)hipFree(dpts); dpts = nullptr; }

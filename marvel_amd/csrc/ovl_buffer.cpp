@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/marvl_align.h"
@@ -39,6 +41,9 @@ int split_block_name(const char* name, std::string& root)
     return atoi(dot + 1);
 }
 
+// bytes of one record on disk
+inline size_t rec_bytes(const Overlap* v, int tbytes) { return 40 + (v->path.trace ? (size_t)v->path.tlen * tbytes : 0); }
+
 bool write_las(const std::string& path, const std::vector<const Overlap*>& recs, size_t lo, size_t hi, int tspace,
                int tbytes)
 {
@@ -47,28 +52,53 @@ bool write_las(const std::string& path, const std::vector<const Overlap*>& recs,
         fprintf(stderr, "[ERROR] - Write_Overlap_Buffer: Cannot open file %s for writing\n", path.c_str());
         exit(1);
     }
-    // one buffered write of the whole file
-    size_t bytes = 12;
-    for (size_t j = lo; j < hi; j++)
-        bytes += 40 + (recs[j]->path.trace ? (size_t)recs[j]->path.tlen * tbytes : 0);
-    std::vector<char> out(bytes);
-    const int64_t n = (int64_t)(hi - lo);
-    memcpy(out.data(), &n, 8);
-    memcpy(out.data() + 8, &tspace, 4);
-    size_t o = 12;
-    for (size_t j = lo; j < hi; j++) {
-        const Overlap* v = recs[j];
-        const int32_t rec[10] = {v->path.tlen,  v->path.diffs,      v->path.abpos, v->path.bbpos, v->path.aepos,
-                                 v->path.bepos, (int32_t)v->flags, v->aread,      v->bread,      0};
-        memcpy(out.data() + o, rec, 40);
-        o += 40;
-        if (v->path.trace) {
-            const size_t tb = (size_t)v->path.tlen * tbytes;
-            memcpy(out.data() + o, v->path.trace, tb);
-            o += tb;
-        }
+    // the file is serialised into one buffer by up to 8 threads, each
+    // filling the slice of a contiguous range of records, then written at once
+    const size_t n = hi - lo;
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n / 50000));
+    std::vector<size_t> cut(nt + 1), start(nt + 1);
+    for (int t = 0; t <= nt; t++) cut[t] = lo + n * t / nt;
+    std::vector<size_t> part(nt, 0);
+    auto sizes = [&](int t) {
+        size_t b = 0;
+        for (size_t j = cut[t]; j < cut[t + 1]; j++) b += rec_bytes(recs[j], tbytes);
+        part[t] = b;
+    };
+    {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(sizes, t);
+        sizes(0);
+        for (auto& x : th) x.join();
     }
-    const bool ok = fwrite(out.data(), 1, bytes, f) == bytes;
+    start[0] = 12;
+    for (int t = 0; t < nt; t++) start[t + 1] = start[t] + part[t];
+    const size_t bytes = start[nt];
+    std::unique_ptr<char[]> out(new char[bytes]);
+    const int64_t nrec = (int64_t)n;
+    memcpy(out.get(), &nrec, 8);
+    memcpy(out.get() + 8, &tspace, 4);
+    auto fill = [&](int t) {
+        size_t o = start[t];
+        for (size_t j = cut[t]; j < cut[t + 1]; j++) {
+            const Overlap* v = recs[j];
+            const int32_t rec[10] = {v->path.tlen,  v->path.diffs,      v->path.abpos, v->path.bbpos, v->path.aepos,
+                                     v->path.bepos, (int32_t)v->flags, v->aread,      v->bread,      0};
+            memcpy(out.get() + o, rec, 40);
+            o += 40;
+            if (v->path.trace) {
+                const size_t tb = (size_t)v->path.tlen * tbytes;
+                memcpy(out.get() + o, v->path.trace, tb);
+                o += tb;
+            }
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(fill, t);
+        fill(0);
+        for (auto& x : th) x.join();
+    }
+    const bool ok = fwrite(out.get(), 1, bytes, f) == bytes;
     fclose(f);
     return ok;
 }

@@ -111,6 +111,11 @@ inline hipError_t hipHostMalloc(T** p, size_t n, unsigned = 0)
     *p = (T*)calloc(1, n ? n : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
+inline hipError_t hipHostFree(void* p)
+{
+    free(p);
+    return hipSuccess;
+}
 template <class T>
 inline hipError_t hipHostGetDevicePointer(T** d, void* h, unsigned)
 {

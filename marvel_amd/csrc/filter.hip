@@ -762,10 +762,14 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     const int cus = mi_num_cus();
     // waves per level: up to 8 per SIMD for the 64-window kernel; the per-wave
     // pools are sized for typical pairs, outliers go to the larger levels
-    const Level levels[4] = {{1, 1 << 15, 128, 1 << 16, cus * 32},
-                             {4, 1 << 17, 256, 1 << 18, cus * 8},
-                             {16, 1 << 17, 256, 1 << 18, cus * 4},
-                             {16, 1 << 20, 4096, 1LL << 22, std::min(cus, 256)}};
+    // aligner windows of 64, 128, 256 and 1,024 diagonals; the last level
+    // also has the large pebble pools and match arenas
+    constexpr int kLevels = 5;
+    const Level levels[kLevels] = {{1, 1 << 15, 128, 1 << 16, cus * 32},
+                                   {2, 1 << 16, 256, 1 << 18, cus * 16},
+                                   {4, 1 << 17, 256, 1 << 18, cus * 8},
+                                   {16, 1 << 17, 256, 1 << 18, cus * 4},
+                                   {16, 1 << 20, 4096, 1LL << 22, std::min(cus, 256)}};
     const int small_trace = TS <= TRACE_XOVR ? 1 : 0;
     const int tbytes = small_trace ? 1 : 2;
 
@@ -932,6 +936,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         g_ev_report.record_start();
         switch (L.nch) {
         case 1: launch_report<1>(b, grid, sc_mode); break;
+        case 2: launch_report<2>(b, grid, sc_mode); break;
         case 4: launch_report<4>(b, grid, sc_mode); break;
         default: launch_report<16>(b, grid, sc_mode); break;
         }
@@ -999,8 +1004,8 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
                 todo.push_back(f.first);
             }
             if (up) {
-                if (level == 3) throw Fail("pairs exceed the largest aligner window / arena");
-                level = arena ? 3 : level + 1;
+                if (level == kLevels - 1) throw Fail("pairs exceed the largest aligner window / arena");
+                level = arena ? kLevels - 1 : level + 1;
             }
             HC(hipMemcpyAsync(order.p, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice, st));
             again = run(order.p, (long long)todo.size(), level);
@@ -1136,7 +1141,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         const int rlast = std::max(A->ufirst + A->nreads, B->ufirst + B->nreads);
         const int rb = bits_for(std::max(1, rlast - rfirst));
         const int pb = bits_for(std::max(A->maxlen, B->maxlen));
-        const int lb = bits_for(std::max(levels[0].mcap, levels[3].mcap));
+        const int lb = bits_for(std::max(levels[0].mcap, levels[kLevels - 1].mcap));
         if (2 * rb + pb + lb > 64) throw Fail("record keys exceed 64 bits", MI_ERR_ARG);
         DBuf<RecKey> k1, k2;
         DBuf<u64> flag, pos;

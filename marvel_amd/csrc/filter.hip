@@ -240,7 +240,7 @@ static Rec* radix_sort(Rec* a, Rec* b, long long n, int lo, int hi, KeyOf key)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_hist<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, n, sh, key, counts, ntiles);
         HC(hipGetLastError());
         g_scan.run(counts, offs, (long long)256 * ntiles);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_scatter<Rec, KeyOf>), dim3(ntiles), dim3(64), 0, st, a, b, n, sh, key, offs, ntiles);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_scatter<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, b, n, sh, key, offs, ntiles);
         HC(hipGetLastError());
         std::swap(a, b);
     }

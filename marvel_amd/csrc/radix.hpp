@@ -7,11 +7,9 @@
 //                   counts[digit * ntiles + tile];
 //   2. scan       : exclusive prefix over counts (digit-major), giving each
 //                   (digit, tile) its output base;
-//   3. rs_scatter : one wave per tile walks the tile in order, 64 records a
-//                   round; lanes with equal digits find each other with 8
-//                   ballots, take ranks from an mbcnt over the peer mask and
-//                   a per-digit LDS cursor, and store to base + rank.  Walking
-//                   the tile in order keeps the sort stable.
+//   3. rs_scatter : one workgroup per tile ranks the tile stably (ballot
+//                   peer groups + per-wave digit counts), stages it in LDS
+//                   in digit order and writes each digit's run contiguously.
 // The key of a record is produced by a functor, so 8-byte keys and 16-byte
 // seed records share the code.
 #pragma once
@@ -39,43 +37,96 @@ __global__ __launch_bounds__(256) void rs_hist(const Rec* __restrict__ in, long 
     counts[(size_t)tid * ntiles + blockIdx.x] = h[tid];   // digit-major: scanning gives each (digit, tile) its base
 }
 
+constexpr int kTileWaves = 4;                        // waves ranking one tile
+constexpr int kTileRounds = kTile / 64 / kTileWaves;   // 64-record rounds per wave
+
+// One 256-thread workgroup per tile.  Wave w ranks records
+// [w*1024, (w+1)*1024) of the tile in order (8 ballots find a lane's peers
+// with the same digit, mbcnt gives its rank among them, a per-wave LDS count
+// per digit carries ranks across rounds); the per-wave counts then give every
+// record its position in the tile sorted by digit, the tile is staged in LDS
+// in that order and written out with consecutive threads storing consecutive
+// records of each digit's run.  Ranks follow input order, so the sort is
+// stable.
 template <class Rec, class KeyOf>
-__global__ __launch_bounds__(64) void rs_scatter(const Rec* __restrict__ in, Rec* __restrict__ out, long long n,
-                                                 int shift, KeyOf key, const unsigned long long* __restrict__ offs,
-                                                 int ntiles)
+__global__ __launch_bounds__(256) void rs_scatter(const Rec* __restrict__ in, Rec* __restrict__ out, long long n,
+                                                  int shift, KeyOf key, const unsigned long long* __restrict__ offs,
+                                                  int ntiles)
 {
-    __shared__ unsigned long long cur[256];
-    const int lane = __lane_id();
-    for (int d = lane; d < 256; d += 64) cur[d] = offs[(size_t)d * ntiles + blockIdx.x];
-    __syncthreads();
+    __shared__ unsigned wcnt[kTileWaves][256];   // per-wave digit counts, then per-wave digit offsets
+    __shared__ unsigned tstart[256];             // first tile position of each digit
+    __shared__ unsigned long long gbase[256];    // output position of the tile's first record of each digit
+    __shared__ Rec buf[kTile];                   // the tile in digit order
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long long base = (long long)blockIdx.x * kTile;
-    for (int r = 0; r < kTile / 64; r++) {
-        const long long g = base + r * 64 + lane;
+#pragma unroll
+    for (int q = 0; q < kTileWaves; q++) wcnt[q][tid] = 0;
+    gbase[tid] = offs[(size_t)tid * ntiles + blockIdx.x];
+    __syncthreads();
+
+    Rec v[kTileRounds];
+    unsigned rk[kTileRounds];
+#pragma unroll
+    for (int r = 0; r < kTileRounds; r++) {
+        const long long g = base + (long long)(w * kTileRounds + r) * 64 + lane;
+        if (g < n) v[r] = in[g];
+    }
+#pragma unroll
+    for (int r = 0; r < kTileRounds; r++) {
+        const long long g = base + (long long)(w * kTileRounds + r) * 64 + lane;
         const bool valid = g < n;
-        Rec v;
-        unsigned dg = 0;
-        if (valid) {
-            v = in[g];
-            dg = (unsigned)(key(v) >> shift) & 255u;
-        }
-        // peers: valid lanes with the same digit
+        const unsigned dg = valid ? (unsigned)(key(v[r]) >> shift) & 255u : 0u;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const unsigned long long m = __ballot((dg >> b) & 1);
             peers &= ((dg >> b) & 1) ? m : ~m;
         }
-        const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0));
-        unsigned long long pos = 0;
-        if (valid) pos = cur[dg] + rank;
-        __syncthreads();   // all lanes read cur before the group leader advances it
-        if (valid) {
-            out[pos] = v;
-            const int last = 63 - __clzll(peers);
-            if (lane == last) cur[dg] += __popcll(peers);
-        }
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0));
+        const unsigned prev = valid ? wcnt[w][dg] : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        __builtin_amdgcn_wave_barrier();   // every lane has read the count before the group's last lane advances it
+        if (valid && lane == 63 - __clzll(peers)) wcnt[w][dg] = prev + (unsigned)__popcll(peers);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        rk[r] = prev + rank;
+    }
+    __syncthreads();
+    // thread tid owns digit tid: per-wave exclusive offsets and the digit's tile total
+    unsigned tot = 0;
+#pragma unroll
+    for (int q = 0; q < kTileWaves; q++) {
+        const unsigned c = wcnt[q][tid];
+        wcnt[q][tid] = tot;
+        tot += c;
+    }
+    // exclusive scan of the 256 digit totals
+    tstart[tid] = tot;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const unsigned a = tid >= o ? tstart[tid - o] : 0u;
         __syncthreads();
-        if (base + (r + 1) * 64 >= n) break;
+        tstart[tid] += a;
+        __syncthreads();
+    }
+    const unsigned incl = tstart[tid];
+    __syncthreads();
+    tstart[tid] = incl - tot;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kTileRounds; r++) {
+        const long long g = base + (long long)(w * kTileRounds + r) * 64 + lane;
+        if (g < n) {
+            const unsigned dg = (unsigned)(key(v[r]) >> shift) & 255u;
+            buf[tstart[dg] + wcnt[w][dg] + rk[r]] = v[r];
+        }
+    }
+    __syncthreads();
+    const int cnt = (int)(n - base < kTile ? n - base : kTile);
+    for (int i = tid; i < cnt; i += 256) {
+        const Rec x = buf[i];
+        const unsigned dg = (unsigned)(key(x) >> shift) & 255u;
+        out[gbase[dg] + (unsigned long long)(i - (int)tstart[dg])] = x;
     }
 }
 

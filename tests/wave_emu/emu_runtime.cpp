@@ -2,7 +2,7 @@
 #include "hip_cpu.hpp"
 
 namespace emu {
-Ctx* g = nullptr;
+thread_local Ctx* g = nullptr;
 thread_local int lane = 0;
 thread_local dim3 t_block, t_thread;
 dim3 g_bdim, g_gdim;

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <functional>
 #include <thread>
+#include <memory>
 #include <vector>
 
 #include "wave_emu.hpp"
@@ -41,9 +42,10 @@ void launch(dim3 grid, dim3 block, F&& body)
     const unsigned nb = grid.x * grid.y * grid.z, nt = block.x * block.y * block.z;
     if (nb == 0 || nt == 0) return;
     YieldBarrier bar((int)nt);
-    Ctx ctx;
+    // wave operations exist for blocks of whole waves: one exchange context per wave
+    const unsigned nw = (nt % 64 == 0) ? nt / 64 : 0;
+    std::unique_ptr<Ctx[]> ctx(new Ctx[nw ? nw : 1]);
     Ctx* saved = g;
-    g = (nt == 64) ? &ctx : nullptr;   // wave operations exist only for one-wave blocks
     g_blockbar = &bar;
     g_bdim = block;
     g_gdim = grid;
@@ -52,6 +54,7 @@ void launch(dim3 grid, dim3 block, F&& body)
     for (unsigned t = 0; t < nt; t++)
         th.emplace_back([&, t] {
             lane = (int)(t & 63);
+            g = nw ? &ctx[t / 64] : nullptr;
             t_thread = dim3(t, 0, 0);
             for (unsigned b = 0; b < nb; b++) {
                 t_block = dim3(b, 0, 0);

@@ -39,7 +39,7 @@ struct Ctx {
     YieldBarrier bar{64};
     long long slot[64];
 };
-extern Ctx* g;
+extern thread_local Ctx* g;   // the calling lane's wave
 extern thread_local int lane;
 template <class T>
 inline T xchg(T v, int src)

@@ -657,7 +657,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     const int MAXGRAM = 10000;
     hist.ensure(MAXGRAM);
     HC(hipMemsetAsync(hist.p, 0, sizeof(u64) * MAXGRAM, st));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_brange), dim3(nblk(ng)), dim3(256), 0, st, A->keys.p, A->fmt, A->gstart.p, ng, B->keys.p, B->fmt, nb, bs.p, be.p);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_brange), dim3(nblk(ng)), dim3(256), 0, st, A->keys.p, A->fmt, A->gstart.p, ng, B->keys.p, B->fmt, nb, same_index ? 1 : 0, bs.p, be.p);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, be.p, B->keys.p, B->fmt, self ? 1 : 0,
                                        cnt.p);
     g_scan.run(cnt.p, off.p, na);

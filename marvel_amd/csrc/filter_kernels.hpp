@@ -167,16 +167,37 @@ __device__ __forceinline__ long long lower_code(const uint64_t* keys, long long 
     return lo;
 }
 
-// per A group: the B range with the same code (find_tuple, dalign/filter.c:911-930)
+// per A group: the B range with the same code (find_tuple, dalign/filter.c:911-930).
+// With B the same index as A the range is the group itself.  Otherwise the
+// start is binary-searched and the end found by galloping from it: most codes
+// have a handful of entries, so the end is in the start's cache line.
 __global__ void k_group_brange(const uint64_t* __restrict__ ak, KeyFmt fa, const long long* __restrict__ gstart,
-                               long long ng, const uint64_t* __restrict__ bk, KeyFmt fb, long long nb,
+                               long long ng, const uint64_t* __restrict__ bk, KeyFmt fb, long long nb, int same,
                                long long* __restrict__ bs, long long* __restrict__ be)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
+    if (same) {
+        bs[g] = gstart[g];
+        be[g] = gstart[g + 1];
+        return;
+    }
     const uint64_t code = fa.code(ak[gstart[g]]);
-    bs[g] = lower_code(bk, nb, code, fb);
-    be[g] = lower_code(bk, nb, code + 1, fb);
+    const long long s = lower_code(bk, nb, code, fb);
+    // first entry past the code: gallop to a bracket [lo, hi), then bisect
+    long long lo = s, step = 1, hi = s;
+    while (hi < nb && fb.code(bk[hi]) == code) {
+        lo = hi + 1;
+        hi = (step < nb - hi) ? hi + step : nb;
+        step <<= 1;
+    }
+    while (lo < hi) {
+        const long long m = (lo + hi) >> 1;
+        if (fb.code(bk[m]) == code) lo = m + 1;
+        else hi = m;
+    }
+    bs[g] = s;
+    be[g] = lo;
 }
 
 // per A entry: number of B entries it pairs with (count_thread,

@@ -226,7 +226,8 @@ static Scanner g_scan;
 
 // ---- stable LSD radix sort over bits [lo, hi) ---------------------------------------
 
-static DBuf<u64> g_rcounts, g_roffs;
+static DBuf<unsigned> g_rcounts;
+static DBuf<u64> g_roffs, g_rtot;
 
 template <class Rec, class KeyOf>
 static Rec* radix_sort(Rec* a, Rec* b, long long n, int lo, int hi, KeyOf key)
@@ -234,12 +235,15 @@ static Rec* radix_sort(Rec* a, Rec* b, long long n, int lo, int hi, KeyOf key)
     if (n <= 1) return a;
     hipStream_t st = fstream();
     const int ntiles = (int)((n + kTile - 1) / kTile);
-    u64* counts = g_rcounts.ensure((size_t)256 * ntiles);
-    u64* offs = g_roffs.ensure((size_t)256 * ntiles + 1);
+    unsigned* counts = g_rcounts.ensure((size_t)256 * ntiles);
+    u64* offs = g_roffs.ensure((size_t)256 * ntiles);
+    u64* tot = g_rtot.ensure(256);
     for (int sh = lo; sh < hi; sh += 8) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_hist<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, n, sh, key, counts, ntiles);
         HC(hipGetLastError());
-        g_scan.run(counts, offs, (long long)256 * ntiles);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_digit_totals), dim3(256), dim3(256), 0, st, counts, ntiles, tot);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_digit_offsets), dim3(256), dim3(256), 0, st, counts, ntiles, tot, offs);
+        HC(hipGetLastError());
         hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_scatter<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, b, n, sh, key, offs, ntiles);
         HC(hipGetLastError());
         std::swap(a, b);

@@ -5,8 +5,9 @@
 //   1. rs_hist    : one 256-thread workgroup per tile of kTile records counts
 //                   the digits of its tile in LDS (LDS atomics) and writes
 //                   counts[digit * ntiles + tile];
-//   2. scan       : exclusive prefix over counts (digit-major), giving each
-//                   (digit, tile) its output base;
+//   2. rs_digit_totals + rs_digit_offsets : each (digit, tile) gets its
+//                   output base: the totals of the smaller digits plus the
+//                   digit's counts in the earlier tiles;
 //   3. rs_scatter : one workgroup per tile ranks the tile stably (ballot
 //                   peer groups + per-wave digit counts), stages it in LDS
 //                   in digit order and writes each digit's run contiguously.
@@ -22,7 +23,7 @@ constexpr int kTile = 4096;   // records per tile (64 lanes x 64 rounds)
 
 template <class Rec, class KeyOf>
 __global__ __launch_bounds__(256) void rs_hist(const Rec* __restrict__ in, long long n, int shift, KeyOf key,
-                                               unsigned long long* __restrict__ counts, int ntiles)
+                                               unsigned* __restrict__ counts, int ntiles)
 {
     __shared__ unsigned h[256];
     const int tid = threadIdx.x;
@@ -127,6 +128,72 @@ __global__ __launch_bounds__(256) void rs_scatter(const Rec* __restrict__ in, Re
         const Rec x = buf[i];
         const unsigned dg = (unsigned)(key(x) >> shift) & 255u;
         out[gbase[dg] + (unsigned long long)(i - (int)tstart[dg])] = x;
+    }
+}
+
+// ---- output bases of a pass: offs[d][t] = Σ_{d' < d} total(d') + Σ_{t' < t} counts[d][t'] ----
+
+// total count of each digit (one workgroup per digit)
+__global__ __launch_bounds__(256) void rs_digit_totals(const unsigned* __restrict__ counts, int ntiles,
+                                                       unsigned long long* __restrict__ tot)
+{
+    __shared__ unsigned long long red[256];
+    const int tid = threadIdx.x;
+    const unsigned* row = counts + (size_t)blockIdx.x * ntiles;
+    unsigned long long s = 0;
+    for (int t = tid; t < ntiles; t += 256) s += row[t];
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) tot[blockIdx.x] = red[0];
+}
+
+// one workgroup per digit: its base from the totals of the smaller digits,
+// then an exclusive scan of its tile counts in chunks of 4,096
+__global__ __launch_bounds__(256) void rs_digit_offsets(const unsigned* __restrict__ counts, int ntiles,
+                                                        const unsigned long long* __restrict__ tot,
+                                                        unsigned long long* __restrict__ offs)
+{
+    __shared__ unsigned long long red[256];
+    __shared__ unsigned sc[256];
+    const int tid = threadIdx.x, d = blockIdx.x;
+    red[tid] = tid < d ? tot[tid] : 0ull;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    unsigned long long carry = red[0];
+    const unsigned* row = counts + (size_t)d * ntiles;
+    unsigned long long* orow = offs + (size_t)d * ntiles;
+    for (int c0 = 0; c0 < ntiles; c0 += 256 * 16) {
+        unsigned v[16], loc = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int t = c0 + tid * 16 + i;
+            v[i] = t < ntiles ? row[t] : 0u;
+            loc += v[i];
+        }
+        sc[tid] = loc;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const unsigned a = tid >= o ? sc[tid - o] : 0u;
+            __syncthreads();
+            sc[tid] += a;
+            __syncthreads();
+        }
+        unsigned long long run = carry + (sc[tid] - loc);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int t = c0 + tid * 16 + i;
+            if (t < ntiles) orow[t] = run;
+            run += v[i];
+        }
+        carry += sc[255];
+        __syncthreads();
     }
 }
 

@@ -7,6 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <memory>
 #include <string>
 #include <thread>
@@ -41,65 +44,112 @@ int split_block_name(const char* name, std::string& root)
     return atoi(dot + 1);
 }
 
+// Stable merge of the sorted ranges a[0, mid) and a[mid, end) in place (tmp:
+// scratch of end entries), split by merge path over up to 8 threads.  On
+// equal keys the first range goes first.
+template <class T, class Less>
+void parallel_merge(T* a, size_t mid, size_t end, T* tmp, Less less)
+{
+    const size_t n1 = mid, n2 = end - mid, n = end;
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n / 65536));
+    // co-rank: the number i of first-range entries among the first k outputs
+    auto corank = [&](size_t k) {
+        size_t lo = k > n2 ? k - n2 : 0, hi = std::min(k, n1);
+        while (lo < hi) {
+            const size_t i = (lo + hi) / 2, j = k - i;
+            // a[i] goes before b[j - 1] unless b[j - 1] < a[i]
+            if (j > 0 && i < n1 && !less(a[mid + j - 1], a[i])) lo = i + 1;
+            else hi = i;
+        }
+        return lo;
+    };
+    std::vector<size_t> ci(nt + 1);
+    for (int t = 0; t <= nt; t++) ci[t] = corank(n * t / nt);
+    auto part = [&](int t) {
+        const size_t k0 = n * t / nt, k1 = n * (t + 1) / nt;
+        const size_t i0 = ci[t], i1 = ci[t + 1];
+        std::merge(a + i0, a + i1, a + mid + (k0 - i0), a + mid + (k1 - i1), tmp + k0, less);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(part, t);
+    part(0);
+    for (auto& x : th) x.join();
+    std::copy(tmp, tmp + n, a);
+}
+
 // bytes of one record on disk
 inline size_t rec_bytes(const Overlap* v, int tbytes) { return 40 + (v->path.trace ? (size_t)v->path.tlen * tbytes : 0); }
+
+// Grow-only scratch kept by the calling thread across calls (the output of
+// a block pair is rebuilt every call; fresh multi-MB buffers would be paid
+// again in page faults each time).
+template <class T>
+T* scratch(std::vector<T>& v, size_t n)
+{
+    if (v.size() < n) v.resize(n + n / 8);
+    return v.data();
+}
 
 bool write_las(const std::string& path, const std::vector<const Overlap*>& recs, size_t lo, size_t hi, int tspace,
                int tbytes)
 {
-    FILE* f = fopen(path.c_str(), "w");
-    if (f == nullptr) {
+    const int fd = open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0666);
+    if (fd < 0) {
         fprintf(stderr, "[ERROR] - Write_Overlap_Buffer: Cannot open file %s for writing\n", path.c_str());
         exit(1);
     }
-    // the file is serialised into one buffer by up to 8 threads, each
-    // filling the slice of a contiguous range of records, then written at once
+    // up to 8 threads each serialise a contiguous range of records into their
+    // slice of one buffer and write that slice at its file offset
     const size_t n = hi - lo;
     const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n / 50000));
-    std::vector<size_t> cut(nt + 1), start(nt + 1);
+    std::vector<size_t> cut(nt + 1), start(nt + 1), part(nt, 0);
     for (int t = 0; t <= nt; t++) cut[t] = lo + n * t / nt;
-    std::vector<size_t> part(nt, 0);
-    auto sizes = [&](int t) {
+    auto run = [&](auto&& fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(fn, t);
+        fn(0);
+        for (auto& x : th) x.join();
+    };
+    run([&](int t) {
         size_t b = 0;
         for (size_t j = cut[t]; j < cut[t + 1]; j++) b += rec_bytes(recs[j], tbytes);
         part[t] = b;
-    };
-    {
-        std::vector<std::thread> th;
-        for (int t = 1; t < nt; t++) th.emplace_back(sizes, t);
-        sizes(0);
-        for (auto& x : th) x.join();
-    }
+    });
     start[0] = 12;
     for (int t = 0; t < nt; t++) start[t + 1] = start[t] + part[t];
     const size_t bytes = start[nt];
-    std::unique_ptr<char[]> out(new char[bytes]);
+    static thread_local std::vector<char> obuf;
+    char* out = scratch(obuf, bytes);
     const int64_t nrec = (int64_t)n;
-    memcpy(out.get(), &nrec, 8);
-    memcpy(out.get() + 8, &tspace, 4);
-    auto fill = [&](int t) {
+    memcpy(out, &nrec, 8);
+    memcpy(out + 8, &tspace, 4);
+    std::vector<char> fail(nt, 0);
+    run([&](int t) {
         size_t o = start[t];
         for (size_t j = cut[t]; j < cut[t + 1]; j++) {
             const Overlap* v = recs[j];
             const int32_t rec[10] = {v->path.tlen,  v->path.diffs,      v->path.abpos, v->path.bbpos, v->path.aepos,
                                      v->path.bepos, (int32_t)v->flags, v->aread,      v->bread,      0};
-            memcpy(out.get() + o, rec, 40);
+            memcpy(out + o, rec, 40);
             o += 40;
             if (v->path.trace) {
                 const size_t tb = (size_t)v->path.tlen * tbytes;
-                memcpy(out.get() + o, v->path.trace, tb);
+                memcpy(out + o, v->path.trace, tb);
                 o += tb;
             }
         }
-    };
-    {
-        std::vector<std::thread> th;
-        for (int t = 1; t < nt; t++) th.emplace_back(fill, t);
-        fill(0);
-        for (auto& x : th) x.join();
-    }
-    const bool ok = fwrite(out.get(), 1, bytes, f) == bytes;
-    fclose(f);
+        const size_t b0 = t == 0 ? 0 : start[t];   // slice 0 also carries the header
+        size_t done = 0, len = start[t + 1] - b0;
+        while (done < len) {
+            const ssize_t w = pwrite(fd, out + b0 + done, len - done, (off_t)(b0 + done));
+            if (w <= 0) {
+                fail[t] = 1;
+                break;
+            }
+            done += (size_t)w;
+        }
+    });
+    const bool ok = close(fd) == 0 && std::find(fail.begin(), fail.end(), 1) == fail.end();
     return ok;
 }
 
@@ -249,34 +299,47 @@ int Symmetric(Align_Spec* espec) { return ((SpecObj*)espec)->symmetric; }
 void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblock, int lastRead)
 {
     SpecObj* s = (SpecObj*)spec;
-    std::vector<const Overlap*> all;
-    size_t n = 0;
-    for (int i = 0; i < s->nthreads; i++) n += s->iobuf[i].otop;
-    all.reserve(n);
-    // thread buffers in order; the stable sort keeps that order for equal keys
-    for (int i = 0; i < s->nthreads; i++)
-        for (int j = 0; j < s->iobuf[i].otop; j++) {
-            const Overlap* o = s->iobuf[i].ovls + j;
-            if (!s->only_identity || o->aread == o->bread) all.push_back(o);
-        }
-    auto less = [](const Overlap* l, const Overlap* r) {
-        if (l->aread != r->aread) return l->aread < r->aread;
-        if (l->bread != r->bread) return l->bread < r->bread;
-        if (COMP(l->flags) != COMP(r->flags)) return COMP(l->flags) < COMP(r->flags);
-        return l->path.abpos < r->path.abpos;
+    // sort keys of all buffered records, thread buffers in order (the stable
+    // order keeps that order for equal keys)
+    struct Key {
+        int aread, bread, comp, abpos;
+        const Overlap* o;
     };
+    auto less = [](const Key& l, const Key& r) {
+        if (l.aread != r.aread) return l.aread < r.aread;
+        if (l.bread != r.bread) return l.bread < r.bread;
+        if (l.comp != r.comp) return l.comp < r.comp;
+        return l.abpos < r.abpos;
+    };
+    static thread_local std::vector<Key> keys, tmp;
+    keys.clear();
+    {
+        size_t n = 0;
+        for (int i = 0; i < s->nthreads; i++) n += s->iobuf[i].otop;
+        keys.reserve(n);
+        for (int i = 0; i < s->nthreads; i++)
+            for (int j = 0; j < s->iobuf[i].otop; j++) {
+                const Overlap* o = s->iobuf[i].ovls + j;
+                if (!s->only_identity || o->aread == o->bread)
+                    keys.push_back(Key{o->aread, o->bread, (int)COMP(o->flags), o->path.abpos, o});
+            }
+    }
     // the buffers usually hold a few already sorted runs (one per filter
     // pass): merge them stably; anything else gets a full stable sort
     std::vector<size_t> runs{0};
-    for (size_t i = 1; i < all.size(); i++)
-        if (less(all[i], all[i - 1])) runs.push_back(i);
+    for (size_t i = 1; i < keys.size(); i++)
+        if (less(keys[i], keys[i - 1])) runs.push_back(i);
     if (runs.size() <= 16) {
+        if (tmp.size() < keys.size()) tmp.resize(keys.size() + keys.size() / 8);
         for (size_t r = 1; r < runs.size(); r++) {
-            const size_t end = (r + 1 < runs.size()) ? runs[r + 1] : all.size();
-            std::inplace_merge(all.begin(), all.begin() + runs[r], all.begin() + end, less);
+            const size_t end = (r + 1 < runs.size()) ? runs[r + 1] : keys.size();
+            parallel_merge(keys.data(), runs[r], end, tmp.data(), less);
         }
     } else
-        std::stable_sort(all.begin(), all.end(), less);
+        std::stable_sort(keys.begin(), keys.end(), less);
+    static thread_local std::vector<const Overlap*> all;
+    all.resize(keys.size());
+    for (size_t i = 0; i < keys.size(); i++) all[i] = keys[i].o;
     std::string aroot, broot;
     const int aid = split_block_name(ablock, aroot);
     const int bid = split_block_name(bblock, broot);

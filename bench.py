@@ -74,7 +74,7 @@ def profiled_traffic(kernel):
             fetch_scale = float(cells[2].split()[0])
         elif len(cells) >= 3 and cells[0] == "WRITE_SIZE (KiB)":
             write_scale = float(cells[2].split()[0])
-        elif len(cells) >= 3 and cells[0] == f"`{kernel}`" and row is None:
+        elif len(cells) >= 3 and cells[0].startswith(f"`{kernel}`") and row is None:
             row = cells
     if row is None or fetch_scale is None or write_scale is None:
         return None
@@ -254,8 +254,8 @@ def main():
             "parity": "bit-exact vs reference daligner" if sha == GOLD_SHA else f"MISMATCH sha256 {sha}",
             "roofline": {"bound": "valu", "kernel": "k_report (one wave64 per read pair: gating + Local_Alignment)",
                          "achieved": round(ach, 2), "peak": VALU_PEAK_GOPS, "unit": "Gop/s (W_align lane-ops)",
-                         "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": profiled_traffic("k_report<1>"),
-                         "traffic_unit": "HBM bytes per k_report<1> launch (rocprofv3 --pmc, calibrated)",
+                         "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": profiled_traffic("k_report<1, 0>"),
+                         "traffic_unit": "HBM bytes per k_report<1, 0> launch (rocprofv3 --pmc, calibrated)",
                          "kernel_ms_per_step": round(k_report * 1e3, 2), "launches_per_step": launches,
                          "w_align_per_step": w_align},
             "roofline_hbm": {"bound": "hbm", "kernel": "seed LSD radix sort (rs_hist + rs_scatter)",

@@ -9,7 +9,7 @@
 // V[low-1] / V[hgh+1].  Neighbour diagonals k-1 / k+1 of the previous wave are
 // fetched with cross-lane gathers (ds_bpermute), the record / trim / clip
 // decisions of the sequential reference loop become wave-wide prefix scans
-// and 64-bit ballots, and match extension ("snake") compares 32 bases per
+// and 64-bit ballots, and match extension ("snake") compares 16 bases per
 // step from 2-bit packed reads.
 //
 // Pebbles (trace-point cells, dalign/align.c:359-365) are appended to a
@@ -71,20 +71,32 @@ __device__ __forceinline__ uint64_t win32(const uint64_t* pk, int p)
     return (lo >> (2 * r)) | (hi << (64 - 2 * r));
 }
 
+// 16-base window starting at base p, read as 32-bit words of the same
+// buffer (base i at bits 2*(i&15) of word i>>4): one funnel shift of the two
+// words holding it.  Windows starting in [-32, len+32] stay inside the
+// guard words.
+__device__ __forceinline__ uint32_t win16(const uint32_t* pk, int p)
+{
+    const int q = p >> 4;
+    return __builtin_amdgcn_alignbit(pk[q + 1], pk[q], (uint32_t)(2 * (p & 15)));
+}
+
 // forward: count of equal bases A[x+i] == B[y+i] for i < lim
 __device__ __forceinline__ int match_fwd(const uint64_t* A, const uint64_t* B, int x, int y, int lim, bool& mism)
 {
+    const uint32_t* a = reinterpret_cast<const uint32_t*>(A);
+    const uint32_t* b = reinterpret_cast<const uint32_t*>(B);
     int s = 0;
     mism = false;
     while (s < lim) {
-        uint64_t d = win32(A, x + s) ^ win32(B, y + s);
+        const uint32_t d = win16(a, x + s) ^ win16(b, y + s);
         if (d) {
-            s += __builtin_ctzll(d) >> 1;
+            s += __builtin_ctz(d) >> 1;
             if (s < lim) mism = true;
             else s = lim;
             return s;
         }
-        s += 32;
+        s += 16;
     }
     return lim;
 }
@@ -92,17 +104,19 @@ __device__ __forceinline__ int match_fwd(const uint64_t* A, const uint64_t* B, i
 // reverse: count of equal bases A[x-1-i] == B[y-1-i] for i < lim
 __device__ __forceinline__ int match_rev(const uint64_t* A, const uint64_t* B, int x, int y, int lim, bool& mism)
 {
+    const uint32_t* a = reinterpret_cast<const uint32_t*>(A);
+    const uint32_t* b = reinterpret_cast<const uint32_t*>(B);
     int s = 0;
     mism = false;
     while (s < lim) {
-        uint64_t d = win32(A, x - s - 32) ^ win32(B, y - s - 32);
+        const uint32_t d = win16(a, x - s - 16) ^ win16(b, y - s - 16);
         if (d) {
-            s += __builtin_clzll(d) >> 1;
+            s += __builtin_clz(d) >> 1;
             if (s < lim) mism = true;
             else s = lim;
             return s;
         }
-        s += 32;
+        s += 16;
     }
     return lim;
 }

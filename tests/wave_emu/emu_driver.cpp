@@ -70,15 +70,16 @@ int main(int argc, char** argv)
         for (int l = 0; l < 64; l++)
             th.emplace_back([&, l] {
                 emu::lane = l;
-                long long cells = 0, snk = 0;
+                emu::g = &ctx;   // the wave context is per thread
+                mi::WaveCount wc;
                 if (nch == 4)
                     res[l] = mi::local_alignment<4>(sp, pa.data() + 1, alen, pb.data() + 1, blen, v[2], v[3], v[4],
                                                     v[5], -INT32_MAX, INT32_MAX, pool.data(), pool_cap, pts.data(),
-                                                    pts_cap, tr.data() + maxtp, tr.data() + 3 * maxtp, maxtp, &g_abort, cells, snk);
+                                                    pts_cap, tr.data() + maxtp, tr.data() + 3 * maxtp, maxtp, &g_abort, wc);
                 else
                     res[l] = mi::local_alignment<1>(sp, pa.data() + 1, alen, pb.data() + 1, blen, v[2], v[3], v[4],
                                                     v[5], -INT32_MAX, INT32_MAX, pool.data(), pool_cap, pts.data(),
-                                                    pts_cap, tr.data() + maxtp, tr.data() + 3 * maxtp, maxtp, &g_abort, cells, snk);
+                                                    pts_cap, tr.data() + maxtp, tr.data() + 3 * maxtp, maxtp, &g_abort, wc);
             });
         for (auto& t : th) t.join();
         const mi::LaResult& r = res[0];

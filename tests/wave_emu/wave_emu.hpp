@@ -110,6 +110,11 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
     return v;
 }
 inline int __builtin_amdgcn_readfirstlane(int v) { return emu::xchg(v, 0); }
+// v_alignbit_b32: low 32 bits of {hi, lo} >> (s & 31)
+inline unsigned __builtin_amdgcn_alignbit(unsigned hi, unsigned lo, unsigned s)
+{
+    return (unsigned)(((((unsigned long long)hi) << 32) | lo) >> (s & 31));
+}
 inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned acc)
 {
     int l = emu::lane < 32 ? emu::lane : 32;

@@ -132,8 +132,9 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
 __device__ __forceinline__ int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 // value of lane (L-1) & 63 / (L+1) & 63: DPP wave_ror:1 / wave_rol:1, full-rate
 // VALU moves (no LDS traffic).  All 64 lanes must be active.
-__device__ __forceinline__ int lane_below(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x13C, 0xF, 0xF, false); }
-__device__ __forceinline__ int lane_above(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false); }
+// (every lane has a source, so no "old" value is needed: one v_mov_b32_dpp)
+__device__ __forceinline__ int lane_below(int v) { return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, false); }
+__device__ __forceinline__ int lane_above(int v) { return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false); }
 __device__ __forceinline__ uint64_t lane_below64(uint64_t v)
 {
     return ((uint64_t)(uint32_t)lane_below((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)lane_below((int)(uint32_t)v);
@@ -143,41 +144,44 @@ __device__ __forceinline__ uint64_t lane_above64(uint64_t v)
     return ((uint64_t)(uint32_t)lane_above((int)(uint32_t)(v >> 32)) << 32) | (uint32_t)lane_above((int)(uint32_t)v);
 }
 // Wave-wide scans from DPP row moves (no LDS): row_shl:n / row_shr:n read
-// lane L+n / L-n of the same 16-lane row (lanes past the row keep the
-// identity), row_bcast:15 / :31 hand the last lane of a row (of two rows)
-// to the next row(s).
+// lane L+n / L-n of the same 16-lane row, row_bcast:15 / :31 hand the last
+// lane of a row (of two rows) to the next row(s).  Lanes without a source
+// read 0 (bound_ctrl), the identity of max over non-negative values, so each
+// level is a single v_max_i32 with a DPP operand.
 template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ int dpp_or(int ident, int v)
+__device__ __forceinline__ int dpp0(int v)
 {
-    return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROW_MASK, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, true);
 }
-// inclusive max over lanes >= L
-__device__ __forceinline__ int wave_suffix_max(int x)
+// inclusive max over lanes >= L of x >= 0
+__device__ __forceinline__ int suffix_max_nn(int x)
 {
-    constexpr int ID = -2147483647 - 1;
-    x = max(x, dpp_or<0x101>(ID, x));   // row_shl:1
-    x = max(x, dpp_or<0x102>(ID, x));   // row_shl:2
-    x = max(x, dpp_or<0x104>(ID, x));   // row_shl:4
-    x = max(x, dpp_or<0x108>(ID, x));   // row_shl:8
+    x = max(x, dpp0<0x101>(x));   // row_shl:1
+    x = max(x, dpp0<0x102>(x));   // row_shl:2
+    x = max(x, dpp0<0x104>(x));   // row_shl:4
+    x = max(x, dpp0<0x108>(x));   // row_shl:8
     // lane 16r now holds row r's maximum
     const int s1 = __builtin_amdgcn_readlane(x, 16), s2 = __builtin_amdgcn_readlane(x, 32);
     const int s3 = __builtin_amdgcn_readlane(x, 48);
     const int t2 = max(s2, s3), t1 = max(s1, t2);
     const int row = __lane_id() >> 4;
-    return max(x, row == 0 ? t1 : (row == 1 ? t2 : (row == 2 ? s3 : ID)));
+    return max(x, row == 0 ? t1 : (row == 1 ? t2 : (row == 2 ? s3 : 0)));
 }
-// inclusive min over lanes <= L
-__device__ __forceinline__ int wave_prefix_min(int x)
+// inclusive max over lanes <= L of x >= 0
+__device__ __forceinline__ int prefix_max_nn(int x)
 {
-    constexpr int ID = 2147483647;
-    x = min(x, dpp_or<0x111>(ID, x));        // row_shr:1
-    x = min(x, dpp_or<0x112>(ID, x));        // row_shr:2
-    x = min(x, dpp_or<0x114>(ID, x));        // row_shr:4
-    x = min(x, dpp_or<0x118>(ID, x));        // row_shr:8
-    x = min(x, dpp_or<0x142, 0xA>(ID, x));   // row_bcast:15 into rows 1, 3
-    x = min(x, dpp_or<0x143, 0xC>(ID, x));   // row_bcast:31 into rows 2, 3
+    x = max(x, dpp0<0x111>(x));        // row_shr:1
+    x = max(x, dpp0<0x112>(x));        // row_shr:2
+    x = max(x, dpp0<0x114>(x));        // row_shr:4
+    x = max(x, dpp0<0x118>(x));        // row_shr:8
+    x = max(x, dpp0<0x142, 0xA>(x));   // row_bcast:15 into rows 1, 3
+    x = max(x, dpp0<0x143, 0xC>(x));   // row_bcast:31 into rows 2, 3
     return x;
 }
+// inclusive max over lanes >= L (x >= -1)
+__device__ __forceinline__ int wave_suffix_max(int x) { return suffix_max_nn(x + 1) - 1; }
+// inclusive min over lanes <= L (any x): the max of INT32_MAX - x >= 0
+__device__ __forceinline__ int wave_prefix_min(int x) { return 2147483647 - prefix_max_nn(2147483647 - x); }
 
 __device__ __forceinline__ uint64_t rdl64(uint64_t v, int lane)
 {

@@ -73,8 +73,9 @@ __global__ __launch_bounds__(64) void st_dpp(const int* in, int* out)
     const int v = in[blockIdx.x * 64 + lane];
     out[(blockIdx.x * 64 + lane) * 4] = lane_below(v);
     out[(blockIdx.x * 64 + lane) * 4 + 1] = lane_above(v);
+    // the aligner's scans: suffix max over values >= -1, prefix min over values >= 0
     out[(blockIdx.x * 64 + lane) * 4 + 2] = wave_suffix_max(v);
-    out[(blockIdx.x * 64 + lane) * 4 + 3] = wave_prefix_min(v);
+    out[(blockIdx.x * 64 + lane) * 4 + 3] = wave_prefix_min(v < 0 ? 2147483647 : v);
 }
 
 }  // namespace mi

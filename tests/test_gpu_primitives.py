@@ -81,21 +81,24 @@ def test_snake_matcher():
 
 def test_dpp_wave_rotations_and_scans():
     """lane_below / lane_above (DPP wave_ror:1 / wave_rol:1) read lanes L-1 /
-    L+1 mod 64; wave_suffix_max / wave_prefix_min (DPP row shifts, row
-    broadcasts, readlane) are the inclusive suffix max / prefix min."""
+    L+1 mod 64; wave_suffix_max (values >= -1, the forward pass's
+    antidiagonals and sentinel) and wave_prefix_min (values >= 0, the reverse
+    pass's, sentinel INT32_MAX) are the inclusive suffix max / prefix min."""
     print("[prim] dpp", flush=True)
     rng = np.random.default_rng(9)
     waves = 8
-    v = rng.integers(-10**6, 10**6, size=waves * 64).astype(np.int32)
+    v = rng.integers(-1, 2**30, size=waves * 64).astype(np.int32)
     v[64:128] = np.arange(64, dtype=np.int32)            # increasing
     v[128:192] = np.arange(64, 0, -1, dtype=np.int32)    # decreasing
     v[192:256] = 7                                       # constant
-    v[256] = np.iinfo(np.int32).max                      # extremes
-    v[300] = np.iinfo(np.int32).min
+    v[256:320] = -1                                      # all sentinels
+    v[320::5] = -1                                       # scattered sentinels
+    v[330] = 2**30
     out = _run(3, inp=v, nout=waves * 64 * 4).reshape(waves, 64, 4)
     for w in range(waves):
         x = v[w * 64:(w + 1) * 64]
+        xr = np.where(x < 0, np.iinfo(np.int32).max, x).astype(np.int32)
         assert np.array_equal(out[w, :, 0], np.roll(x, 1))
         assert np.array_equal(out[w, :, 1], np.roll(x, -1))
         assert np.array_equal(out[w, :, 2], np.maximum.accumulate(x[::-1])[::-1])
-        assert np.array_equal(out[w, :, 3], np.minimum.accumulate(x))
+        assert np.array_equal(out[w, :, 3], np.minimum.accumulate(xr))

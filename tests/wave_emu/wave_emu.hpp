@@ -88,8 +88,8 @@ inline int __any(int p) { return __ballot(p) != 0; }
 inline int __builtin_amdgcn_readlane(int v, int l) { return emu::xchg(v, l); }
 // DPP moves: 0x13C wave_ror:1, 0x134 wave_rol:1, 0x138 wave_shr:1, 0x130 wave_shl:1,
 // 0x101-0x10F row_shl:n, 0x111-0x11F row_shr:n, 0x142 row_bcast:15, 0x143 row_bcast:31
-// (bank_mask 0xF and bound_ctrl false only: lanes without a source keep `old`)
-inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int, bool)
+// (bank_mask 0xF only; lanes without a source read 0 with bound_ctrl, else keep `old`)
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int, bool bound_ctrl)
 {
     const int l = emu::lane, row = l >> 4, rl = l & 15;
     int from = -1;
@@ -106,8 +106,13 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
         else __builtin_trap();
     }
     const int v = emu::xchg(src, from < 0 ? l : from);   // every lane takes part in the exchange
-    if (!((row_mask >> row) & 1) || from < 0) return old;
+    if (!((row_mask >> row) & 1)) return old;
+    if (from < 0) return bound_ctrl ? 0 : old;
     return v;
+}
+inline int __builtin_amdgcn_mov_dpp(int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl)
+{
+    return __builtin_amdgcn_update_dpp(0, src, ctrl, row_mask, bank_mask, bound_ctrl);
 }
 inline int __builtin_amdgcn_readfirstlane(int v) { return emu::xchg(v, 0); }
 // v_alignbit_b32: low 32 bits of {hi, lo} >> (s & 31)

@@ -47,8 +47,8 @@ constexpr int LA_CHAIN = 4;   // internal consistency check of a pebble chain fa
 constexpr int LA_ABORT = 5;   // host watchdog asked the kernel to stop
 
 struct DevSpec {
-    const int16_t* table;   // [32768] suffix-positivity table
-    const int16_t* score;   // [32768]
+    const int16_t* table;   // [32768] suffix-positivity table, then the [32768] score table (one base register)
+    const int16_t* score;   // == table + 32768
     int tspace;
     int ave_path;
 };
@@ -160,12 +160,14 @@ __device__ __forceinline__ int suffix_max_nn(int x)
     x = max(x, dpp0<0x102>(x));   // row_shl:2
     x = max(x, dpp0<0x104>(x));   // row_shl:4
     x = max(x, dpp0<0x108>(x));   // row_shl:8
-    // lane 16r now holds row r's maximum
+    // lane 16r now holds row r's maximum; lanes of row r add the rows above
+    // it, selected with arithmetic masks (-1 / 0) rather than lane masks
     const int s1 = __builtin_amdgcn_readlane(x, 16), s2 = __builtin_amdgcn_readlane(x, 32);
     const int s3 = __builtin_amdgcn_readlane(x, 48);
-    const int t2 = max(s2, s3), t1 = max(s1, t2);
-    const int row = __lane_id() >> 4;
-    return max(x, row == 0 ? t1 : (row == 1 ? t2 : (row == 2 ? s3 : 0)));
+    int row = __lane_id() >> 4;
+    asm volatile("" : "+v"(row));   // keeps the row masks from being hoisted into (spilled) SGPR lane masks
+    const int m1 = (row - 1) >> 31, m2 = (row - 2) >> 31, m3 = (row - 3) >> 31;
+    return max(max(x, s1 & m1), max(s2 & m2, s3 & m3));
 }
 // inclusive max over lanes <= L of x >= 0
 __device__ __forceinline__ int prefix_max_nn(int x)
@@ -178,6 +180,10 @@ __device__ __forceinline__ int prefix_max_nn(int x)
     x = max(x, dpp0<0x143, 0xC>(x));   // row_bcast:31 into rows 2, 3
     return x;
 }
+// value of lane L+1 / L-1; the lane without one (63 / 0) gets `edge`
+// (DPP wave_shl:1 / wave_shr:1)
+__device__ __forceinline__ int lane_above_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int lane_below_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xF, 0xF, false); }
 // inclusive max over lanes >= L (x >= -1)
 __device__ __forceinline__ int wave_suffix_max(int x) { return suffix_max_nn(x + 1) - 1; }
 // inclusive min over lanes <= L (any x): the max of INT32_MAX - x >= 0
@@ -447,9 +453,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 const int j = (DIR > 0) ? (NCH - 1 - jj) : jj;
                 const int c = cN[j];
                 const int incl = (DIR > 0) ? wave_suffix_max(c) : wave_prefix_min(c);
-                int excl = (DIR > 0) ? lane_above(incl) : lane_below(incl);
-                if (DIR > 0 && lane == 63) excl = SENT;
-                if (DIR < 0 && lane == 0) excl = SENT;
+                const int excl = (DIR > 0) ? lane_above_or(incl, SENT) : lane_below_or(incl, SENT);
                 const int prior = (DIR > 0) ? max(excl, carry) : min(excl, carry);
                 const bool rec = (DIR > 0) ? (c > prior) : (c < prior);
                 bool okm, oktr = false;
@@ -461,7 +465,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                     if (okm) {
                         const uint64_t b = bN[j];
                         if (sp.table[b & kTrimMask] >= 0)
-                            oktr = (sp.table[(b >> kTrimLen) & kTrimMask] + sp.score[b & kTrimMask] >= 0);
+                            oktr = (sp.table[(b >> kTrimLen) & kTrimMask] + sp.table[32768 + (b & kTrimMask)] >= 0);
                     }
                 }
                 const unsigned long long brec = __ballot(rec), bm = __ballot(okm), bt = __ballot(oktr);

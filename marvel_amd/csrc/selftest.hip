@@ -71,11 +71,14 @@ __global__ __launch_bounds__(64) void st_dpp(const int* in, int* out)
 {
     const int lane = __lane_id();
     const int v = in[blockIdx.x * 64 + lane];
-    out[(blockIdx.x * 64 + lane) * 4] = lane_below(v);
-    out[(blockIdx.x * 64 + lane) * 4 + 1] = lane_above(v);
+    int* o = out + (blockIdx.x * 64 + lane) * 6;
+    o[0] = lane_below(v);
+    o[1] = lane_above(v);
     // the aligner's scans: suffix max over values >= -1, prefix min over values >= 0
-    out[(blockIdx.x * 64 + lane) * 4 + 2] = wave_suffix_max(v);
-    out[(blockIdx.x * 64 + lane) * 4 + 3] = wave_prefix_min(v < 0 ? 2147483647 : v);
+    o[2] = wave_suffix_max(v);
+    o[3] = wave_prefix_min(v < 0 ? 2147483647 : v);
+    o[4] = lane_above_or(v, -7);
+    o[5] = lane_below_or(v, -9);
 }
 
 }  // namespace mi

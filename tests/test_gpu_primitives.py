@@ -94,7 +94,7 @@ def test_dpp_wave_rotations_and_scans():
     v[256:320] = -1                                      # all sentinels
     v[320::5] = -1                                       # scattered sentinels
     v[330] = 2**30
-    out = _run(3, inp=v, nout=waves * 64 * 4).reshape(waves, 64, 4)
+    out = _run(3, inp=v, nout=waves * 64 * 6).reshape(waves, 64, 6)
     for w in range(waves):
         x = v[w * 64:(w + 1) * 64]
         xr = np.where(x < 0, np.iinfo(np.int32).max, x).astype(np.int32)
@@ -102,3 +102,5 @@ def test_dpp_wave_rotations_and_scans():
         assert np.array_equal(out[w, :, 1], np.roll(x, -1))
         assert np.array_equal(out[w, :, 2], np.maximum.accumulate(x[::-1])[::-1])
         assert np.array_equal(out[w, :, 3], np.minimum.accumulate(xr))
+        assert np.array_equal(out[w, :, 4], np.append(x[1:], -7))   # wave_shl:1, lane 63 keeps the edge value
+        assert np.array_equal(out[w, :, 5], np.insert(x[:-1], 0, -9))   # wave_shr:1, lane 0 keeps it

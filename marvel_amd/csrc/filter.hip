@@ -618,7 +618,7 @@ __global__ __launch_bounds__(64) void k_trace_gather(OutRec* __restrict__ recs, 
 
 // candidate scorer state (scorer.hip)
 int scorer_mode();
-bool scorer_weights(MlpW* w, float* thr);
+const MlpGate* scorer_gate();
 void scorer_capture_append(const float* feat, const int* label, long long n);
 
 template <int NCH>
@@ -875,7 +875,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     a.ocount = ocount.p;
     // candidate scorer: gate with the loaded weights, or capture features + outcomes
     const int sc_mode = scorer_mode();
-    if (sc_mode == SC_GATE && !scorer_weights(&a.mlp, &a.mlp_thr))
+    if (sc_mode == SC_GATE && (a.gate = scorer_gate()) == nullptr)
         throw Fail("candidate gating requested without scorer weights", MI_ERR_ARG);
     DBuf<float> capf;
     DBuf<int> capm;

@@ -90,8 +90,7 @@ struct ReportArgs {
     unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events, [4] wave steps,
                                   // [5] candidates scored, [6] candidates skipped by the scorer
     // candidate scorer (report kernel instantiations with SC != 0)
-    MlpW mlp;                     // SC == 2: weights
-    float mlp_thr;                //          align when logit >= mlp_thr
+    const MlpGate* gate;          // SC == 2: weights and threshold (align when logit >= thr)
     float* cap_feat;              // SC == 1: kFeat features per captured candidate
     int* cap_meta;                //          4 ints: kept, job, launch, 0
     unsigned long long* cap_count;
@@ -246,7 +245,7 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
 // measured on the 5 Mbp / 30x workload 3 -> 4 -> 5 -> 6 -> 8 waves per SIMD
 // took the kernel from 380 to 305, 277, 258 and 248 ms per step.
 template <int NCH, int SC>
-__global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
+__global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report(ReportArgs a)
 {
     const int lane = __lane_id();
     const int wave = blockIdx.x;
@@ -382,16 +381,16 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                     }
                     if constexpr (SC == SC_GATE) {
                         __shared__ __attribute__((aligned(16))) uint16_t mlp_lds[2 * kMlpLdsHalves];
-                        // weight pointers made opaque here so that their loads stay
-                        // inside this block instead of being hoisted out of the seed
-                        // loop (they would hold ~56 VGPRs across every alignment)
-                        MlpW wl = a.mlp;
-                        asm volatile("" : "+v"(wl.w1), "+v"(wl.w2), "+v"(wl.w3), "+v"(wl.b1), "+v"(wl.b2));
-                        const f32x4 o = mlp_tile(cand_operand(feats), wl, mlp_lds);
+                        // the gate is read here (the opaque pointer keeps the loads from
+                        // being hoisted out of the seed loop and held across alignments)
+                        const MlpGate* gp = a.gate;
+                        asm volatile("" : "+v"(gp));
+                        const MlpW w = gp->w;
+                        const f32x4 o = mlp_tile(cand_operand(feats), w, mlp_lds);
                         const float logit = __builtin_bit_cast(
-                            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, o[0]), 0)) + a.mlp.b3;
+                            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, o[0]), 0)) + w.b3;
                         scored++;
-                        if (logit < a.mlp_thr) {   // predicted not to yield a kept overlap
+                        if (logit < gp->thr) {   // predicted not to yield a kept overlap
                             skipped++;
                             continue;
                         }

@@ -21,6 +21,7 @@ struct Scorer {
     void* dev = nullptr;     // packed weights + biases, one allocation
     MlpW w{};
     float thr = 0.f;
+    MlpGate* dgate = nullptr;   // device copy of {w, thr} for the report kernel
     std::vector<float> cap_feat;   // captured candidates (kFeat floats each)
     std::vector<int> cap_label;
 };
@@ -90,13 +91,10 @@ __global__ __launch_bounds__(256) void k_mlp_batch(const float* __restrict__ fea
 
 int scorer_mode() { return S().mode; }
 
-bool scorer_weights(MlpW* w, float* thr)
+const MlpGate* scorer_gate()
 {
     Scorer& s = S();
-    if (!s.loaded) return false;
-    *w = s.w;
-    *thr = s.thr;
-    return true;
+    return s.loaded ? s.dgate : nullptr;
 }
 
 void scorer_capture_append(const float* feat, const int* label, long long n)
@@ -146,6 +144,12 @@ int mi_scorer_load(const float* w1, const float* b1, const float* w2, const floa
     s.w.b2 = s.w.b1 + kHid;
     s.w.b3 = b3;
     s.thr = threshold;
+    const MlpGate g{s.w, threshold};
+    if ((s.dgate == nullptr && hipMalloc(&s.dgate, sizeof(MlpGate)) != hipSuccess) ||
+        hipMemcpy(s.dgate, &g, sizeof g, hipMemcpyHostToDevice) != hipSuccess) {
+        mi_set_error("mi_scorer_load: copy to the device failed");
+        return MI_ERR_DEVICE;
+    }
     s.loaded = true;
     return 0;
 }

@@ -165,6 +165,17 @@ __device__ __forceinline__ void cand_features(const CandIn& c, float (&f)[kFeat]
     f[15] = lg16((float)max(end, 0));
 }
 
+// Logit of one candidate (row 0 of a tile).  Not inlined: the report
+// kernel's register allocation around the aligner stays free of the MLP's
+// temporaries.  All 64 lanes must be active.
+
+// Weights and threshold of the gate, in device memory: the report kernel
+// reads them where it scores a candidate instead of keeping them live.
+struct MlpGate {
+    MlpW w;
+    float thr;
+};
+
 // Layer-1 A operand of a tile whose row 0 is this candidate (other rows zero).
 __device__ __forceinline__ bf16x8 cand_operand(const float (&f)[kFeat])
 {

@@ -15,6 +15,7 @@
 #define __device__
 #define __global__
 #define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
 #define __host__
 
 namespace emu {

@@ -143,6 +143,24 @@ def config_small_repeats(seed: int = 11):
     return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
 
 
+def config_band_edges(seed: int = 21):
+    """Reads that drive report_thread's outermost diagonal bands: two reads of
+    the block's maximum length X1, X2 (4,990 bp); Y begins with X1's last 60 bp
+    (the pair A=Y, B=X1 lies in the lowest band, whose widened span reaches
+    the band-array cell shared with the highest band's lastp); W begins with
+    X2's last 60 bp (the later pair A=X2, B=W scores seeds in the highest
+    band); error-free, with random filler reads around them.  Read order
+    R0 < X1 < Y < W < X2 < R5.."""
+    rng = np.random.default_rng(seed)
+    rnd = lambda n: rng.integers(0, 4, size=n, dtype=np.uint8)
+    x1, x2 = rnd(4990), rnd(4990)
+    y = np.concatenate([x1[-60:], rnd(2940)])
+    w = np.concatenate([x2[-60:], rnd(2940)])
+    reads = [rnd(3000), x1, y, w, x2] + [rnd(int(rng.integers(1500, 4000))) for _ in range(6)]
+    n = len(reads)
+    return ReadSet(reads, np.zeros(n, dtype=np.int64), np.array([len(r) for r in reads]), np.zeros(n, dtype=np.int64))
+
+
 def config_5mbp(seed: int = 2):
     """5 Mbp iid genome, 30x, lognormal(10 kb, 6 kb, min 1 kb), 9/4/2 % errors."""
     g = random_genome(5_000_000, seed)

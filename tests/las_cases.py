@@ -19,6 +19,7 @@ DATA = {
     "small": lambda: sim.config_small(1),
     "small_rep": lambda: sim.config_small_repeats(11),
     "5mbp": lambda: sim.config_5mbp(2),
+    "edges": lambda: sim.config_band_edges(21),
 }
 
 # name -> (data, DBsplit size in Mbp, daligner flags, block arguments)
@@ -37,6 +38,9 @@ CASES = {
     "rep_j8": ("small_rep", 400, ["-j8", "-M16"], ["C1.1", "C1.1"]),
     "rep_t30": ("small_rep", 400, ["-t30", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "rep_j16_l500": ("small_rep", 400, ["-j16", "-l500", "-M16"], ["C1.1", "C1.1"]),
+    # pairs in the outermost diagonal bands (the band cell shared by lastp[maxdiag] and lasta[mindiag-1])
+    "edge_j1": ("edges", 400, ["-j1", "-M16"], ["E.1", "E.1"]),
+    "edge_j4": ("edges", 400, ["-j4", "-M16"], ["E.1", "E.1"]),
     # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)
     "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
 }

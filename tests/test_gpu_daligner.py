@@ -99,3 +99,31 @@ def test_reference_driver_on_library(case):
         r = subprocess.run([INTEG] + flags + blocks, cwd=d, capture_output=True, text=True, timeout=240)
         assert r.returncode == 0, r.stderr[-2000:]
         assert _shas(d) == GOLD[case]["las"]
+
+
+@pytest.mark.gpu
+def test_band_edge_cell_rerun(tmp_path):
+    """Pairs in the outermost diagonal bands: an alignment in the lowest band
+    raises the band-array cell that is also the highest band's lastp, and the
+    pairs of the report chunk after it are re-run with the carried value
+    (edge events and a fix-up must occur), with the reference's .las."""
+    from marvel_amd.daligner import Daligner
+
+    case = "edge_j1"
+    _, _, flags, blocks = las_cases.CASES[case]
+    print(f"[gpu daligner] {case} in process", flush=True)
+    las_cases.make_db(case, str(tmp_path))
+    dal = Daligner()
+    dal.configure(nthreads=1, mem_limit_gb=16, minover=2000)
+    dal.set_filter_params(14, 6, 0, 35)
+    A = dal.open_block(str(tmp_path / blocks[0]))
+    spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=1)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        s1, s2 = dal.compare_self(A, spec, 1, blocks[0])
+    finally:
+        os.chdir(cwd)
+    assert _shas(str(tmp_path)) == GOLD[case]["las"]
+    assert s1["edge_events"] + s2["edge_events"] > 0
+    assert s1["edge_fixups"] + s2["edge_fixups"] >= 1

@@ -826,8 +826,11 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         HC(hipGetLastError());
     }
 
-    // output arenas: about two records per kept pair; overflows grow them
+    // output arenas: about two records per kept pair; overflows grow them and
+    // re-run the pairs that did not fit.  MI_REPORT_ARENA (records) sets the
+    // initial size instead, e.g. a tiny one to exercise that path.
     long long ocap = std::max<long long>(4096, 4 * nfirst);
+    if (const char* e = getenv("MI_REPORT_ARENA")) ocap = std::max(16LL, atoll(e));
     long long otcap = ocap * 160;
     DBuf<OutRec> orecs;
     DBuf<uint8_t> otr;

@@ -127,3 +127,30 @@ def test_band_edge_cell_rerun(tmp_path):
     assert _shas(str(tmp_path)) == GOLD[case]["las"]
     assert s1["edge_events"] + s2["edge_events"] > 0
     assert s1["edge_fixups"] + s2["edge_fixups"] >= 1
+
+
+@pytest.mark.gpu
+def test_output_arena_overflow_reruns(tmp_path, monkeypatch):
+    """A tiny initial output arena: pairs whose records do not fit are re-run
+    after the arena grows, and only each pair's last complete run counts."""
+    from marvel_amd.daligner import Daligner
+
+    case = "c1_j8"
+    _, _, flags, blocks = las_cases.CASES[case]
+    print(f"[gpu daligner] {case} with a 64-record initial arena", flush=True)
+    monkeypatch.setenv("MI_REPORT_ARENA", "64")
+    las_cases.make_db(case, str(tmp_path))
+    dal = Daligner()
+    dal.configure(nthreads=8, mem_limit_gb=16, minover=2000)
+    dal.set_filter_params(14, 6, 0, 35)
+    A = dal.open_block(str(tmp_path / blocks[0]))
+    spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=8)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        s1, s2 = dal.compare_self(A, spec, 1, blocks[0])
+    finally:
+        os.chdir(cwd)
+    assert s1["retries"] + s2["retries"] > 0
+    assert s1["report_launches"] + s2["report_launches"] > 2
+    assert _shas(str(tmp_path)) == GOLD[case]["las"]

@@ -997,9 +997,12 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     };
 
     // runs a job list until every job is done: output-arena overflows are
-    // re-run at the same window, wide bands / full pools at the next one
+    // re-run at the same window, wide bands / full pools at the next one.
+    // MI_REPORT_FIRST_LEVEL starts every job at a wider window instead.
+    int first_level = 0;
+    if (const char* e = getenv("MI_REPORT_FIRST_LEVEL")) first_level = std::min(std::max(atoi(e), 0), kLevels - 1);
     auto run_all = [&](const int* dorder, long long n) {
-        int level = 0;
+        int level = first_level;
         std::vector<std::pair<int, int>> again = run(dorder, n, level);
         while (!again.empty()) {
             S.retries += (long long)again.size();

@@ -154,3 +154,31 @@ def test_output_arena_overflow_reruns(tmp_path, monkeypatch):
     assert s1["retries"] + s2["retries"] > 0
     assert s1["report_launches"] + s2["report_launches"] > 2
     assert _shas(str(tmp_path)) == GOLD[case]["las"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [1, 2, 3, 4])
+def test_every_report_window_matches_reference(tmp_path, monkeypatch, level):
+    """Every pair started at aligner window level 1-4 (128 / 256 / 1,024
+    diagonals, then the large pools and arenas) writes the reference's .las:
+    the wide report kernels, otherwise run only for the few pairs that
+    outgrow the 64-lane window, on a whole workload."""
+    from marvel_amd.daligner import Daligner
+
+    case = "rep_j8"
+    _, _, flags, blocks = las_cases.CASES[case]
+    print(f"[gpu daligner] {case} from window level {level}", flush=True)
+    monkeypatch.setenv("MI_REPORT_FIRST_LEVEL", str(level))
+    las_cases.make_db(case, str(tmp_path))
+    dal = Daligner()
+    dal.configure(nthreads=8, mem_limit_gb=16, minover=2000)
+    dal.set_filter_params(14, 6, 0, 35)
+    A = dal.open_block(str(tmp_path / blocks[0]))
+    spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=8)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        dal.compare_self(A, spec, 1, blocks[0])
+    finally:
+        os.chdir(cwd)
+    assert _shas(str(tmp_path)) == GOLD[case]["las"]

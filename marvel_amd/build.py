@@ -56,6 +56,10 @@ def build(verbose: bool = False) -> str:
           os.path.join(CSRC, CLI_SOURCE), "-o", exe + ".tmp", "-L", HERE, "-lmarvl_align",
           "-Wl,-rpath,$ORIGIN/.."])
     os.replace(exe + ".tmp", exe)
+    # the block-pair plan emitter (host only)
+    hpc = os.path.join(bindir, "HPCdaligner")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "hpcdaligner_main.cpp"), "-o", hpc + ".tmp"])
+    os.replace(hpc + ".tmp", hpc)
     return out
 
 

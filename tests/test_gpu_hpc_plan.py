@@ -1,0 +1,43 @@
+"""A block-pair plan run end to end: marvel_amd/bin/HPCdaligner plans the
+small read set split into 3 blocks, the GPU daligner runs every daligner
+line of the plan, and all nine .las files equal the reference daligner's run
+of the reference plan (tests/golden/hpc/plan_run.json, from
+tests/golden/make_plan_goldens.py).  The reference plan passes -M16 on as
+-s16, so the run also covers 16-base trace spacing."""
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+
+import make_plan_goldens as mp  # noqa: E402
+
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "hpc", "plan_run.json")))
+HPC = os.path.join(ROOT, "marvel_amd", "bin", "HPCdaligner")
+DAL = os.path.join(ROOT, "marvel_amd", "bin", "daligner")
+
+
+@pytest.mark.gpu
+def test_plan_run_matches_reference(tmp_path):
+    print("[hpc plan] 3 blocks", flush=True)
+    d = str(tmp_path)
+    mp.make_db(d)
+    lines = mp.daligner_lines(HPC, d)
+    assert lines == GOLD["daligner_lines"]
+    for args in lines:
+        print("[hpc plan] daligner " + " ".join(args), flush=True)
+        r = subprocess.run([DAL] + args, cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+    got = {}
+    for dp, _, fs in os.walk(d):
+        for f in fs:
+            if f.endswith(".las"):
+                p = os.path.join(dp, f)
+                got[os.path.relpath(p, d)] = hashlib.sha256(open(p, "rb").read()).hexdigest()
+    assert got == GOLD["las"]

@@ -29,7 +29,21 @@ typedef struct {
     int flags;
 } HITS_READ;
 
-/* db/DB.h:213-234 (tracks are not used by the filter) */
+/* db/DB.h:179-185.  The filter reads the first record of HITS_DB.tracks as
+ * the soft mask of daligner -m, as read_DB leaves it
+ * (dalign/daligner.c:373-431): anno = int64[nreads+1] offsets counted in
+ * points (the file's byte offsets divided by 4), data = int32 points, read
+ * i's points at data[anno[i] .. anno[i+1]) in pairs (begin, end) of masked
+ * intervals, non-decreasing and within [0, rlen]. */
+typedef struct HITS_TRACK_ {
+    struct HITS_TRACK_ *next;
+    char *name;
+    int size;
+    void *anno;
+    void *data;
+} HITS_TRACK;
+
+/* db/DB.h:213-234 */
 typedef struct {
     int ureads;
     float freq[4];
@@ -42,7 +56,7 @@ typedef struct {
     int loaded;
     void *bases;        /* bytes 0..3 per base, with a 4 before and after every read */
     HITS_READ *reads;   /* [0..nreads) */
-    void *tracks;
+    void *tracks;       /* HITS_TRACK list; the first one masks k-mers (-m) */
 } HITS_DB;
 
 /* Driver-owned parameters read by the filter (dalign/filter.h:54-73,
@@ -66,8 +80,11 @@ int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin);
 /* Sort_Kmers: replaces dalign/filter.c:698-903 (dalign/filter.h:69).
  * Builds the sorted k-mer index of `block` on the GPU and returns an opaque
  * handle (the block's 2-bit packed reads stay on the device with it);
- * *len = number of k-mers after -t suppression.  BIASED (-b) is not
- * supported.  The handle is released by Match_Filter when passed as its
+ * *len = number of k-mers after -t suppression.  With block->tracks set,
+ * only k-mers wholly outside the first track's intervals are indexed
+ * (tuple_thread's masked branch, dalign/filter.c:416-470); a track whose
+ * points are not increasing pairs within the reads is an error.  BIASED
+ * (-b) is not supported.  The handle is released by Match_Filter when passed as its
  * btable with atable != btable (the reference frees its B index there,
  * dalign/filter.c:2194-2202), otherwise by mi_free_kmer_index. */
 void *Sort_Kmers(HITS_DB *block, int *len);
@@ -156,8 +173,25 @@ int mi_open_block(const char *path, HITS_DB *db);
 void mi_close_block(HITS_DB *db);
 /* Header of the block's reverse complement for mi_sort_kmers_complement
  * (read records shared with src, no bases, freq swapped as complement_DB
- * swaps it, dalign/daligner.c:456-573). */
+ * swaps it, dalign/daligner.c:456-573).  When src has a mask track, dst gets
+ * its complement (each read's points reversed, x -> rlen - x,
+ * dalign/daligner.c:519-560), released by mi_release_tracks(dst). */
 void mi_complement_header(const HITS_DB *src, HITS_DB *dst);
+
+/* The soft-mask tracks of daligner -m for a block opened by mi_open_block:
+ * read_DB's track part (dalign/daligner.c:373-431).  Each name is loaded
+ * from the MARVEL layout (.ROOT.NAME.a2/.d2, zlib chunks; lib/tracks.c:19-170)
+ * or else the Dazzler one (.ROOT.<block>.NAME.anno/.data, then
+ * .ROOT.NAME.anno/.data from the block's first read on; db/DB.c:967-1120);
+ * n > 1 tracks are merged into their union (Merge_Tracks,
+ * dalign/daligner.c:275-370).  db->tracks is set to the one resulting track.
+ * The reference trims a MARVEL-layout track of a block whose first read is
+ * beyond 1 twice, reading outside the block's slice; this loader takes the
+ * block's own slice.  Returns 0 or MI_ERR_ARG (message in mi_last_error). */
+int mi_load_mask_tracks(HITS_DB *db, const char *const *names, int n);
+/* Frees the tracks this library allocated for db (mi_load_mask_tracks,
+ * mi_complement_header) and clears db->tracks; mi_close_block calls it. */
+void mi_release_tracks(HITS_DB *db);
 
 #ifdef __cplusplus
 }

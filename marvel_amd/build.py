@@ -17,7 +17,7 @@ DEV_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
 LIB_SOURCES = ["la_batch.hip", "filter.hip", "trace_pts.hip", "scorer.hip", "selftest.hip", "spec.cpp", "align_capi.cpp", "ovl_buffer.cpp",
-               "db_block.cpp"]
+               "db_block.cpp", "tracks.cpp"]
 CLI_SOURCE = "daligner_main.cpp"
 
 
@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> str:
         objs.append(o)
     out = os.path.join(HERE, "libmarvl_align.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, out)
     # the GPU daligner driver, linked against the library next to it
     bindir = os.path.join(HERE, "bin")

@@ -4,12 +4,14 @@
 //
 //   daligner [-vAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]
 //            [-e<double(.70)>] [-l<int(1000)>] [-s<int(100)>] [-H<int>]
-//            [-r<int(1)>] [-j<int(4)>] <subject:db|dam> <target:db|dam> ...
+//            [-r<int(1)>] [-j<int(4)>] [-m<track>]+ <subject:db> <target:db> ...
 //
 // -j keeps its reference meaning (the number of report chunks, a power of
 // 2), which decides the pairs the reference skips at chunk ends; the GPU
-// work itself is not split by it.  -m (mask tracks), -b (biased k-mers),
-// -I / -O (identity) and .dam inputs are not supported.
+// work itself is not split by it.  -m<track> (repeatable) soft-masks the
+// intervals of the named tracks, Dazzler (.anno/.data) or MARVEL
+// (.a2/.d2) layout.  -b (biased k-mers), -I / -O (identity) and .dam inputs
+// are not supported.
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -18,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/marvl_align.h"
 #include "../../include/marvl_filter.h"
@@ -49,6 +52,7 @@ static void usage()
     fprintf(stderr,
             "usage: daligner [-vAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]\n"
             "                [-e<double(.70)>] [-l<int(1000)>] [-s<int(100)>] [-H<int>] [-r<int(1)>] [-j<int(4)>]\n"
+            "                [-m<track>]+\n"
             "                <subject:db> <target:db> ...\n");
 }
 
@@ -69,6 +73,7 @@ int main(int argc, char** argv)
 {
     int kmer = 14, binshift = 6, hitmin = 35, suppress = 0, tspace = 100, run_id = 1, no_trace = 0, l = 1000;
     double ave_err = .70;
+    std::vector<const char*> masks;
     VERBOSE = 0;
     SYMMETRIC = 1;
     HGAP_MIN = 0;
@@ -99,6 +104,7 @@ int main(int argc, char** argv)
         case 'l': l = atoi(optarg); break;
         case 's': tspace = atoi(optarg); break;
         case 'r': run_id = atoi(optarg); break;
+        case 'm': masks.push_back(optarg); break;
         case 'M': {
             const int t = atoi(optarg);
             if (t < 0) fprintf(stderr, "invalid memory limit of (%d)\n", t);
@@ -142,6 +148,11 @@ int main(int argc, char** argv)
         fprintf(stderr, "daligner: cannot open %s: %s\n", afile, mi_last_error());
         return 1;
     }
+    // read_DB: the block's -m tracks, merged when there are several (dalign/daligner.c:373-431)
+    if (!masks.empty() && mi_load_mask_tracks(&A, masks.data(), (int)masks.size())) {
+        printf("[ERROR] - Unable to load track: %s\n", mi_last_error());
+        return 1;
+    }
     for (int i = 0; i < A.nreads; i++)
         if (A.reads[i].rlen < kmer) {
             fprintf(stderr, "[ERROR] - daligner: Block %s contains reads < %dbp long !  Run DBsplit.\n", afile, kmer);
@@ -182,6 +193,7 @@ int main(int argc, char** argv)
             }
             const double t1 = now_s();
             Match_Filter(aname, &A, aname, &cA, aindex, 0, cindex, 0, 1, spec);
+            mi_release_tracks(&cA);
             if (VERBOSE) print_stats("complement", now_s() - t1);
             Write_Overlap_Buffer(spec, run_id, aname, aname, A.ufirst + A.nreads - 1);
             Reset_Overlap_Buffer(spec);
@@ -189,6 +201,10 @@ int main(int argc, char** argv)
             HITS_DB B;
             if (mi_open_block(bfile, &B)) {
                 fprintf(stderr, "daligner: cannot open %s: %s\n", bfile, mi_last_error());
+                return 1;
+            }
+            if (!masks.empty() && mi_load_mask_tracks(&B, masks.data(), (int)masks.size())) {
+                printf("[ERROR] - Unable to load track: %s\n", mi_last_error());
                 return 1;
             }
             for (int r = 0; r < B.nreads; r++)
@@ -211,6 +227,7 @@ int main(int argc, char** argv)
             if (VERBOSE) print_stats("forward", now_s() - t0);
             const double t1 = now_s();
             Match_Filter(aname, &A, bname, &cB, aindex, 0, cindex, 0, 1, spec);
+            mi_release_tracks(&cB);
             if (VERBOSE) print_stats("complement", now_s() - t1);
             const int last = (B.part < A.part) ? B.ufirst + B.nreads - 1 : A.ufirst + A.nreads - 1;
             Write_Overlap_Buffer(spec, run_id, aname, bname, last);

@@ -17,6 +17,13 @@
 #include "../../include/marvl_filter.h"
 #include "mi_common.h"
 
+namespace mi {
+// Complement of the mask track of block src (its first track) as a new
+// library-owned HITS_TRACK for the complemented block (tracks.cpp); nullptr
+// when src has no track or memory runs out.
+void* complement_track(const HITS_DB* src);
+}
+
 namespace {
 
 bool read_stub(const std::string& stub, int part, int* ufirst, int* ulast)
@@ -172,6 +179,7 @@ int mi_open_block(const char* path, HITS_DB* db)
 
 void mi_close_block(HITS_DB* db)
 {
+    mi_release_tracks(db);
     if (db->reads) free(db->reads - 1);
     free(db->bases);
     free(db->path);
@@ -181,7 +189,9 @@ void mi_close_block(HITS_DB* db)
 // Header of the reverse-complemented block (dalign/daligner.c:456-573):
 // same reads, frequencies of A<->T and C<->G swapped.  The read records
 // are shared with `src`; bases are not produced (the GPU index of the
-// complement is derived from the source index).
+// complement is derived from the source index).  A mask track of src is
+// complemented for dst; like complement_DB, running out of memory there
+// ends the process.
 void mi_complement_header(const HITS_DB* src, HITS_DB* dst)
 {
     *dst = *src;
@@ -191,6 +201,14 @@ void mi_complement_header(const HITS_DB* src, HITS_DB* dst)
     dst->freq[3] = src->freq[0];
     dst->bases = nullptr;
     dst->path = nullptr;
+    dst->tracks = nullptr;
+    if (src->tracks) {
+        dst->tracks = mi::complement_track(src);
+        if (!dst->tracks) {
+            fprintf(stderr, "mi_complement_header: cannot allocate the complemented mask track\n");
+            exit(1);
+        }
+    }
 }
 
 }  // extern "C"

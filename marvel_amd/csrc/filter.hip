@@ -263,6 +263,9 @@ struct KIdx {
     long long nwords = 0;
     std::vector<int> rlen;
     std::vector<long long> w0;
+    // soft mask (-m): read i's points at mpts[manno[i] .. manno[i+1]); empty when unmasked
+    std::vector<long long> manno;
+    std::vector<int> mpts;
     DBuf<uint64_t> keys, pk;
     DBuf<long long> dw0;
     DBuf<int> dlen;
@@ -298,6 +301,35 @@ static void layout_reads(KIdx* x, const HITS_DB* blk)
     x->fmt.rbits = bits_for(std::max(1, x->nreads - 1));
 }
 
+// The block's mask track (the first of blk->tracks, as read_DB leaves it:
+// offsets in points) copied into the index, checked to be increasing pairs
+// within the reads (other tracks make tuple_thread read outside the read,
+// dalign/filter.c:424-452).
+static void take_mask(KIdx* x, const HITS_DB* blk)
+{
+    x->manno.clear();
+    x->mpts.clear();
+    const HITS_TRACK* t = (const HITS_TRACK*)blk->tracks;
+    if (t == nullptr) return;
+    if (t->anno == nullptr || (t->data == nullptr && ((const int64_t*)t->anno)[x->nreads] > ((const int64_t*)t->anno)[0]))
+        throw Fail("mask track without offsets or points", MI_ERR_ARG);
+    const int64_t* an = (const int64_t*)t->anno;
+    const int* pt = (const int*)t->data;
+    x->manno.resize(x->nreads + 1);
+    for (int i = 0; i <= x->nreads; i++) x->manno[i] = an[i] - an[0];
+    x->mpts.assign(pt ? pt + an[0] : nullptr, pt ? pt + an[x->nreads] : nullptr);
+    for (int i = 0; i < x->nreads; i++) {
+        const long long b = x->manno[i], f = x->manno[i + 1];
+        if (f < b || ((f - b) & 1)) throw Fail("mask track: odd number of points for a read", MI_ERR_ARG);
+        int prev = 0;
+        for (long long k = b; k < f; k++) {
+            if (x->mpts[k] < prev || x->mpts[k] > x->rlen[i])
+                throw Fail("mask track: points not increasing within the read", MI_ERR_ARG);
+            prev = x->mpts[k];
+        }
+    }
+}
+
 static void build_keys(KIdx* x)
 {
     hipStream_t st = fstream();
@@ -305,21 +337,52 @@ static void build_keys(KIdx* x)
     x->K = K;
     if (2 * K + x->fmt.rbits + x->fmt.pbits > 64)
         throw Fail("block too large for 64-bit k-mer keys (2k + read bits + position bits > 64)", MI_ERR_ARG);
-    std::vector<long long> kofs(x->nreads + 1);
+    // runs of consecutive k-mers: one per read, or with a mask one per gap
+    // between masked intervals that holds a k-mer, [0, pt0), [pt1, pt2), ...,
+    // [ptlast, rlen) (dalign/filter.c:416-452); kofs[s] = k-mers before run s
+    std::vector<long long> kofs;
+    std::vector<int> sread, spos0;
     long long nk = 0;
-    for (int i = 0; i < x->nreads; i++) {
-        kofs[i] = nk;
-        nk += std::max(0, x->rlen[i] - K + 1);
+    const bool masked = !x->manno.empty();
+    if (!masked) {
+        kofs.resize(x->nreads + 1);
+        for (int i = 0; i < x->nreads; i++) {
+            kofs[i] = nk;
+            nk += std::max(0, x->rlen[i] - K + 1);
+        }
+    } else {
+        for (int i = 0; i < x->nreads; i++) {
+            const long long b = x->manno[i], f = x->manno[i + 1];
+            for (long long a = b; a <= f; a += 2) {
+                const int p = (a == b) ? 0 : x->mpts[a - 1];
+                const int q = (a == f) ? x->rlen[i] : x->mpts[a];
+                if (p + K > q) continue;
+                kofs.push_back(nk);
+                sread.push_back(i);
+                spos0.push_back(p + K - 1);
+                nk += q - p - K + 1;
+            }
+        }
+        kofs.push_back(nk);
     }
-    kofs[x->nreads] = nk;
+    const int nrun = (int)kofs.size() - 1;
+    kofs.back() = nk;
     DBuf<long long> dk;
-    dk.ensure(x->nreads + 1);
-    HC(hipMemcpyAsync(dk.p, kofs.data(), sizeof(long long) * (x->nreads + 1), hipMemcpyHostToDevice, st));
+    DBuf<int> dsr, dsp;
+    dk.ensure(nrun + 1);
+    HC(hipMemcpyAsync(dk.p, kofs.data(), sizeof(long long) * (nrun + 1), hipMemcpyHostToDevice, st));
+    if (masked && nrun > 0) {
+        dsr.ensure(nrun);
+        dsp.ensure(nrun);
+        HC(hipMemcpyAsync(dsr.p, sread.data(), sizeof(int) * nrun, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(dsp.p, spos0.data(), sizeof(int) * nrun, hipMemcpyHostToDevice, st));
+    }
     DBuf<uint64_t> k2;
     x->keys.ensure(nk + 1);
     k2.ensure(nk + 1);
     if (nk > 0) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_kmers), dim3(nblk(nk)), dim3(256), 0, st, x->pk.p, x->dw0.p, dk.p, x->nreads, nk, K, x->fmt, x->keys.p);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_kmers), dim3(nblk(nk)), dim3(256), 0, st, x->pk.p, x->dw0.p, dk.p, nrun,
+                           masked ? dsr.p : nullptr, masked ? dsp.p : nullptr, nk, K, x->fmt, x->keys.p);
         HC(hipGetLastError());
     }
     const int cs = x->fmt.cshift();
@@ -386,6 +449,7 @@ static KIdx* index_from_block(HITS_DB* blk, bool keys = true)
         HC(hipGetLastError());
         HC(hipStreamSynchronize(st));
     }
+    take_mask(x.get(), blk);
     if (keys) build_keys(x.get());
     return x.release();
 }
@@ -401,6 +465,8 @@ static KIdx* index_from_device(const KIdx* src)
     x->fmt = src->fmt;
     x->rlen = src->rlen;
     x->w0 = src->w0;
+    x->manno = src->manno;
+    x->mpts = src->mpts;
     x->nwords = src->nwords;
     hipStream_t st = fstream();
     x->pk.ensure(x->nwords);
@@ -426,6 +492,7 @@ static KIdx* index_complement(const KIdx* src, HITS_DB* cblk)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_revcomp), dim3(x->nreads), dim3(256), 0, st, src->pk.p, x->pk.p, x->dw0.p, x->dlen.p, x->nreads);
         HC(hipGetLastError());
     }
+    take_mask(x.get(), cblk);
     build_keys(x.get());
     return x.release();
 }

@@ -85,23 +85,25 @@ __global__ void k_revcomp(const uint64_t* __restrict__ in, uint64_t* __restrict_
     }
 }
 
-// one thread per k-mer: index -> (read, rpos) -> key.  kofs[r] = k-mers
-// before read r; rpos is the position of the k-mer's last base
-// (dalign/filter.c:400-490).
+// one thread per k-mer: index -> (read, rpos) -> key.  The k-mers come in
+// runs of consecutive positions: kofs[s] = k-mers before run s; run s is
+// read seg_read[s] from last-base position seg_pos0[s] on (without a mask,
+// seg_read == nullptr: run r is all of read r from K-1 on).  rpos is the
+// position of the k-mer's last base (dalign/filter.c:400-490).
 __global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __restrict__ w0,
-                        const long long* __restrict__ kofs, int nreads, long long nk, int K, KeyFmt fmt,
-                        uint64_t* __restrict__ keys)
+                        const long long* __restrict__ kofs, int nrun, const int* __restrict__ seg_read,
+                        const int* __restrict__ seg_pos0, long long nk, int K, KeyFmt fmt, uint64_t* __restrict__ keys)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nk) return;
-    int lo = 0, hi = nreads;   // last read r with kofs[r] <= i
+    int lo = 0, hi = nrun;   // last run s with kofs[s] <= i
     while (hi - lo > 1) {
         const int m = (lo + hi) >> 1;
         if (kofs[m] <= i) lo = m;
         else hi = m;
     }
-    const int r = lo;
-    const int rpos = (int)(i - kofs[r]) + K - 1;
+    const int r = seg_read ? seg_read[lo] : lo;
+    const int rpos = (int)(i - kofs[lo]) + (seg_pos0 ? seg_pos0[lo] : K - 1);
     const uint64_t* w = pk + w0[r];
     // bases rpos-K+1 .. rpos, first base most significant
     const int b0 = rpos - K + 1;

@@ -20,6 +20,8 @@ import ctypes as C
 import os
 import time
 
+import numpy as np
+
 from .lib import LIB_PATH, LibraryMissing
 
 
@@ -36,6 +38,14 @@ class HITS_DB(C.Structure):     # db/DB.h:213-234
                 ("nreads", C.c_int), ("part", C.c_int), ("ufirst", C.c_int), ("path", C.c_char_p),
                 ("loaded", C.c_int), ("bases", C.c_void_p), ("reads", C.POINTER(HITS_READ)),
                 ("tracks", C.c_void_p)]
+
+
+class HITS_TRACK(C.Structure):  # db/DB.h:179-185
+    pass
+
+
+HITS_TRACK._fields_ = [("next", C.POINTER(HITS_TRACK)), ("name", C.c_char_p), ("size", C.c_int),
+                       ("anno", C.c_void_p), ("data", C.c_void_p)]
 
 
 class MiFilterStats(C.Structure):
@@ -76,6 +86,8 @@ class Daligner:
             "mi_open_block": ([C.c_char_p, db], i),
             "mi_close_block": ([db], None),
             "mi_complement_header": ([db, db], None),
+            "mi_load_mask_tracks": ([db, C.POINTER(C.c_char_p), i], i),
+            "mi_release_tracks": ([db], None),
             "Write_Overlap_Buffer": ([vp, i, C.c_char_p, C.c_char_p, i], None),
             "Reset_Overlap_Buffer": ([vp], None),
             "mi_last_error": ([], C.c_char_p),
@@ -114,6 +126,27 @@ class Daligner:
 
     def close_block(self, b: HITS_DB):
         self.h.mi_close_block(C.byref(b))
+
+    def load_mask_tracks(self, b: HITS_DB, names):
+        """read_DB's -m tracks (dalign/daligner.c:373-431): b.tracks becomes
+        their union (one track)."""
+        arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        if self.h.mi_load_mask_tracks(C.byref(b), arr, len(names)):
+            self._err("load mask tracks")
+
+    def release_tracks(self, b: HITS_DB):
+        self.h.mi_release_tracks(C.byref(b))
+
+    @staticmethod
+    def mask_points(b: HITS_DB):
+        """Per read the point list of b's mask track (None without one)."""
+        if not b.tracks:
+            return None
+        t = C.cast(b.tracks, C.POINTER(HITS_TRACK)).contents
+        anno = np.ctypeslib.as_array(C.cast(t.anno, C.POINTER(C.c_int64)), shape=(b.nreads + 1,)).copy()
+        n = int(anno[-1] - anno[0])
+        pts = np.ctypeslib.as_array(C.cast(t.data, C.POINTER(C.c_int32)), shape=(max(n, 1),))[:n].copy()
+        return [pts[anno[i] - anno[0]:anno[i + 1] - anno[0]] for i in range(b.nreads)]
 
     def new_align_spec(self, ave_corr, tspace, freq, nthreads=4, symmetric=1, only_identity=0, no_trace=0):
         f = (C.c_float * 4)(*freq)
@@ -184,11 +217,16 @@ class Daligner:
         try:
             s1 = self.match_filter(nm, A, nm, A, a, a, 0, spec)
             t.append(time.perf_counter())
-            cA = cA if cA is not None else self.complement_header(A)
-            c = self.sort_kmers_complement(a, cA)
-            t.append(time.perf_counter())
-            s2 = self.match_filter(nm, A, nm, cA, a, c, 1, spec)   # consumes c
-            t.append(time.perf_counter())
+            own = cA is None
+            cA = self.complement_header(A) if own else cA
+            try:
+                c = self.sort_kmers_complement(a, cA)
+                t.append(time.perf_counter())
+                s2 = self.match_filter(nm, A, nm, cA, a, c, 1, spec)   # consumes c
+                t.append(time.perf_counter())
+            finally:
+                if own:
+                    self.release_tracks(cA)   # the complemented mask, if A has one
         finally:
             self.free_index(a)
         self.write_overlap_buffer(spec, run_id, nm, nm, A.ufirst + A.nreads - 1)

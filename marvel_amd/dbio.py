@@ -100,6 +100,90 @@ def read_idx(root: str):
     return {"ureads": ureads, "freq": [f0, f1, f2, f3], "maxlen": maxlen, "totlen": totlen}, rec
 
 
+# ---- interval tracks (the -m masks of daligner) -------------------------------
+#
+# Two on-disk layouts, both named after the DB's hidden prefix (".ROOT."):
+#   Dazzler  .ROOT.NAME.anno: int nreads, int size (= 8), int64 byte offsets
+#            [nreads+1] into .ROOT.NAME.data, which holds int32 points
+#            (written by db/DBdust.c:195-216, read by Load_Track,
+#            db/DB.c:967-1120).  A per-block track is .ROOT.<b>.NAME.anno.
+#   MARVEL   .ROOT.NAME.a2: a 64-byte header (u16 version = 2, u16 size = 8,
+#            u32 pad, u64 len = nreads, u64 clen, u64 cdlen, 4 x u64 reserved)
+#            then the anno offsets in zlib chunks; .ROOT.NAME.d2: the points
+#            in zlib chunks.  A chunk is a u64 compressed length followed by
+#            the zlib stream of at most 8 MiB of input (lib/tracks.h:40-56,
+#            lib/tracks.c:60-170, lib/compression.c:11-100).
+# A read's points come in pairs (begin, end) of masked intervals.
+
+TRACK_CHUNK = 8 * 1024 * 1024
+
+
+def _zchunks(raw: bytes) -> bytes:
+    import zlib
+    out = []
+    for o in range(0, len(raw), TRACK_CHUNK):
+        c = zlib.compress(raw[o:o + TRACK_CHUNK])
+        out.append(struct.pack("<Q", len(c)) + c)
+    return b"".join(out)
+
+
+def _unzchunks(raw: bytes) -> bytes:
+    import zlib
+    out, o = [], 0
+    while o < len(raw):
+        (n,) = struct.unpack_from("<Q", raw, o)
+        out.append(zlib.decompress(raw[o + 8:o + 8 + n]))
+        o += 8 + n
+    return b"".join(out)
+
+
+def write_track(root: str, name: str, points, fmt: str = "dazzler", block: int | None = None) -> None:
+    """Writes interval track NAME of DB ROOT; points[i] = the flat point list
+    (b0, e0, b1, e1, ...) of read i.  fmt "dazzler" or "marvel"; block = a
+    per-block Dazzler track (.ROOT.<block>.NAME.*)."""
+    d, base = os.path.split(root)
+    d = d or "."
+    stem = "." + base + (f".{block}" if block else "") + "." + name
+    data = np.concatenate([np.asarray(p, dtype=np.int32) for p in points]) if len(points) else np.zeros(0, np.int32)
+    anno = np.zeros(len(points) + 1, dtype=np.int64)
+    anno[1:] = np.cumsum([4 * len(p) for p in points])
+    if fmt == "dazzler":
+        with open(os.path.join(d, stem + ".anno"), "wb") as f:
+            f.write(struct.pack("<ii", len(points), 8) + anno.tobytes())
+        with open(os.path.join(d, stem + ".data"), "wb") as f:
+            f.write(data.tobytes())
+    elif fmt == "marvel":
+        if block:
+            raise ValueError("MARVEL tracks are whole-DB tracks")
+        ca, cd = _zchunks(anno.tobytes()), _zchunks(data.tobytes())
+        hdr = struct.pack("<HHIQQQ4Q", 2, 8, 0, len(points), len(ca), len(cd), 0, 0, 0, 0)
+        with open(os.path.join(d, stem + ".a2"), "wb") as f:
+            f.write(hdr + ca)
+        with open(os.path.join(d, stem + ".d2"), "wb") as f:
+            f.write(cd)
+    else:
+        raise ValueError(f"unknown track format {fmt!r}")
+
+
+def read_track(root: str, name: str):
+    """Point lists of every read of a whole-DB track (either layout)."""
+    d, base = os.path.split(root)
+    stem = os.path.join(d or ".", "." + base + "." + name)
+    if os.path.exists(stem + ".a2"):
+        raw = open(stem + ".a2", "rb").read()
+        _, _, _, n, clen, cdlen = struct.unpack_from("<HHIQQQ", raw, 0)
+        anno = np.frombuffer(_unzchunks(raw[64:64 + clen]), dtype=np.int64)
+        data = np.frombuffer(_unzchunks(open(stem + ".d2", "rb").read()[:cdlen]), dtype=np.int32)
+    else:
+        raw = open(stem + ".anno", "rb").read()
+        n, size = struct.unpack_from("<ii", raw, 0)
+        if size != 8:
+            raise ValueError(f"{stem}.anno: annotation size {size}, expected 8")
+        anno = np.frombuffer(raw, dtype=np.int64, count=n + 1, offset=8)
+        data = np.fromfile(stem + ".data", dtype=np.int32)
+    return [data[anno[i] // 4:anno[i + 1] // 4].copy() for i in range(n)]
+
+
 # ---- .las (dalign/align.c:3325-3350) -----------------------------------------
 
 LAS_REC = np.dtype([("tlen", "<i4"), ("diffs", "<i4"), ("abpos", "<i4"), ("bbpos", "<i4"), ("aepos", "<i4"),

@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <zlib.h>
 
 /* ------------------------------------------------------------------------ */
 /* Constants of the adaptive wave (dalign/align.c:145-159).                 */
@@ -930,6 +931,9 @@ int or_block_load(or_block *b, const char *path)
     free(pk);
     free(foff);
     fclose(bps);
+    b->ureads = ureads;
+    b->stem = (char *)xrealloc(NULL, strlen(dir) + strlen(root) + 3);
+    sprintf(b->stem, "%s/.%s", dir, root);
     return 0;
 }
 
@@ -938,6 +942,9 @@ void or_block_free(or_block *b)
     free(b->rlen);
     free(b->boff);
     free(b->alloc);
+    free(b->stem);
+    free(b->manno);
+    free(b->mdata);
     memset(b, 0, sizeof(*b));
 }
 
@@ -969,6 +976,247 @@ void or_block_complement(const or_block *src, or_block *dst)
         }
         if (l == r) s[l] = (char)(3 - s[l]);
     }
+    /* the complemented mask: each read's points reversed and mapped x -> rlen - x
+     * (dalign/daligner.c:519-560) */
+    dst->stem = NULL;
+    dst->manno = NULL;
+    dst->mdata = NULL;
+    if (src->manno) {
+        int64_t nd = src->manno[src->nreads];
+        dst->manno = (int64_t *)xrealloc(NULL, sizeof(int64_t) * (src->nreads + 1));
+        dst->mdata = (int32_t *)xrealloc(NULL, sizeof(int32_t) * (nd + 1));
+        memcpy(dst->manno, src->manno, sizeof(int64_t) * (src->nreads + 1));
+        for (i = 0; i < src->nreads; i++)
+            for (int64_t k = src->manno[i], j = src->manno[i + 1] - 1; k < src->manno[i + 1]; k++, j--)
+                dst->mdata[k] = src->rlen[i] - src->mdata[j];
+    }
+}
+
+/* ======================================================================== */
+/* Mask tracks of daligner -m: read_DB / Merge_Size / Merge_Tracks
+ * (dalign/daligner.c:193-431), Load_Track (db/DB.c:967-1120), track_load /
+ * track_load_block (lib/tracks.c:19-170), uncompress_chunks
+ * (lib/compression.c:64-100).  A track comes back as anno[nreads+1] (in
+ * points, as read_DB leaves it after dividing the byte offsets by 4) and
+ * the block's slice of the points. */
+
+static int read_file(const char *path, unsigned char **buf, int64_t *len)
+{
+    FILE *f = fopen(path, "rb");
+    if (f == NULL) return -1;
+    fseeko(f, 0, SEEK_END);
+    *len = ftello(f);
+    fseeko(f, 0, SEEK_SET);
+    *buf = (unsigned char *)xrealloc(NULL, *len + 1);
+    int ok = *len == 0 || fread(*buf, *len, 1, f) == 1;
+    fclose(f);
+    return ok ? 0 : -1;
+}
+
+/* zlib chunks: u64 compressed length, then the stream of <= 8 MiB of input */
+static int unchunk(const unsigned char *in, int64_t len, unsigned char *out, int64_t olen)
+{
+    int64_t i = 0, o = 0;
+    while (i < len) {
+        uint64_t c;
+        if (i + 8 > len) return -1;
+        memcpy(&c, in + i, 8);
+        if ((int64_t)c > len - i - 8) return -1;
+        uLongf d = (uLongf)(olen - o);
+        if (uncompress(out + o, &d, in + i + 8, (uLong)c) != Z_OK) return -1;
+        i += 8 + (int64_t)c;
+        o += (int64_t)d;
+    }
+    return o == olen ? 0 : -1;
+}
+
+/* MARVEL layout .ROOT.NAME.a2 / .d2.  The block's slice is
+ * anno[ufirst .. ufirst+nreads] rebased to 0.  The reference trims a block
+ * with ufirst > 1 twice (track_load at lib/tracks.c:151-163, then
+ * track_load_block at :40-58), which reads outside the block's slice; the
+ * single trim here is the intended one. */
+static int load_marvel(const or_block *b, const char *name, int64_t **anno, int32_t **data, char *err, int el)
+{
+    char fn[8192];
+    unsigned char *a = NULL, *d = NULL;
+    int64_t alen, dlen;
+    snprintf(fn, sizeof fn, "%s.%s.a2", b->stem, name);
+    if (read_file(fn, &a, &alen)) return 1;   /* absent: the caller tries the Dazzler layout */
+    uint16_t size;
+    uint64_t len, clen, cdlen;
+    if (alen < 64) { snprintf(err, el, "%s: short header", fn); free(a); return -1; }
+    memcpy(&size, a + 2, 2);
+    memcpy(&len, a + 8, 8);
+    memcpy(&clen, a + 16, 8);
+    memcpy(&cdlen, a + 24, 8);
+    if (size != 8 || (b->part == 0 && len != (uint64_t)b->nreads) || len < (uint64_t)(b->ufirst + b->nreads) ||
+        (int64_t)clen > alen - 64) {
+        snprintf(err, el, "%s: invalid header", fn);
+        free(a);
+        return -1;
+    }
+    int64_t *full = (int64_t *)xrealloc(NULL, 8 * (len + 1));
+    if (unchunk(a + 64, (int64_t)clen, (unsigned char *)full, 8 * (int64_t)(len + 1))) {
+        snprintf(err, el, "%s: bad compressed offsets", fn);
+        free(a); free(full);
+        return -1;
+    }
+    free(a);
+    snprintf(fn, sizeof fn, "%s.%s.d2", b->stem, name);
+    int64_t tot = full[len];
+    unsigned char *all = (unsigned char *)xrealloc(NULL, tot + 4);
+    if (tot > 0 && (read_file(fn, &d, &dlen) || (int64_t)cdlen > dlen || unchunk(d, (int64_t)cdlen, all, tot))) {
+        snprintf(err, el, "%s: missing or bad point data", fn);
+        free(d); free(all); free(full);
+        return -1;
+    }
+    free(d);
+    int64_t lo = full[b->ufirst];
+    *anno = (int64_t *)xrealloc(NULL, sizeof(int64_t) * (b->nreads + 1));
+    for (int i = 0; i <= b->nreads; i++) (*anno)[i] = (full[b->ufirst + i] - lo) / 4;
+    int64_t np = (*anno)[b->nreads];
+    *data = (int32_t *)xrealloc(NULL, sizeof(int32_t) * (np + 1));
+    memcpy(*data, all + lo, 4 * np);
+    free(all);
+    free(full);
+    return 0;
+}
+
+/* Dazzler layout: .ROOT.<part>.NAME.anno/.data when present, else the
+ * whole-DB .ROOT.NAME.anno/.data read from the block's first read on. */
+static int load_dazzler(const or_block *b, const char *name, int64_t **anno, int32_t **data, char *err, int el)
+{
+    char fn[8192];
+    FILE *af = NULL, *df;
+    int ispart = 0, tlen, size;
+    if (b->part > 0) {
+        snprintf(fn, sizeof fn, "%s.%d.%s.anno", b->stem, b->part, name);
+        af = fopen(fn, "rb");
+        ispart = af != NULL;
+    }
+    if (af == NULL) {
+        snprintf(fn, sizeof fn, "%s.%s.anno", b->stem, name);
+        af = fopen(fn, "rb");
+    }
+    if (af == NULL) { snprintf(err, el, "track %s does not exist", name); return -1; }
+    if (fread(&tlen, 4, 1, af) != 1 || fread(&size, 4, 1, af) != 1 || size != 8 ||
+        tlen != (ispart ? b->nreads : b->ureads)) {
+        snprintf(err, el, "%s: junk or wrong size annotation", fn);
+        fclose(af);
+        return -1;
+    }
+    if (!ispart && b->part > 0) fseeko(af, 8LL * b->ufirst, SEEK_CUR);
+    int64_t *an = (int64_t *)xrealloc(NULL, sizeof(int64_t) * (b->nreads + 1));
+    if (fread(an, 8, b->nreads + 1, af) != (size_t)(b->nreads + 1)) {
+        snprintf(err, el, "%s: short annotation", fn);
+        fclose(af); free(an);
+        return -1;
+    }
+    fclose(af);
+    if (ispart) snprintf(fn, sizeof fn, "%s.%d.%s.data", b->stem, b->part, name);
+    else snprintf(fn, sizeof fn, "%s.%s.data", b->stem, name);
+    int64_t off = an[0];
+    for (int i = 0; i <= b->nreads; i++) an[i] = (an[i] - off) / 4;
+    int64_t np = an[b->nreads];
+    int32_t *dt = (int32_t *)xrealloc(NULL, sizeof(int32_t) * (np + 1));
+    df = fopen(fn, "rb");
+    if (df == NULL || fseeko(df, off, SEEK_SET) != 0 || (np > 0 && fread(dt, 4 * np, 1, df) != 1)) {
+        snprintf(err, el, "%s: missing or short point data", fn);
+        if (df) fclose(df);
+        free(an); free(dt);
+        return -1;
+    }
+    fclose(df);
+    *anno = an;
+    *data = dt;
+    return 0;
+}
+
+/* A track the extraction can use: per read an even number of points, non
+ * decreasing, within [0, rlen].  (Other tracks make tuple_thread read
+ * outside the read, dalign/filter.c:424-452.) */
+static int check_points(const or_block *b, const int64_t *anno, const int32_t *data)
+{
+    for (int i = 0; i < b->nreads; i++) {
+        if (anno[i + 1] < anno[i] || ((anno[i + 1] - anno[i]) & 1)) return -1;
+        int prev = 0;
+        for (int64_t k = anno[i]; k < anno[i + 1]; k++) {
+            if (data[k] < prev || data[k] > b->rlen[i]) return -1;
+            prev = data[k];
+        }
+    }
+    return 0;
+}
+
+int or_block_load_masks(or_block *b, char *const *names, int n, char *err, int errlen)
+{
+    int64_t **an = (int64_t **)xrealloc(NULL, sizeof(int64_t *) * (n + 1));
+    int32_t **dt = (int32_t **)xrealloc(NULL, sizeof(int32_t *) * (n + 1));
+    int t, rc = 0;
+    for (t = 0; t < n; t++) {
+        for (int u = 0; u < t; u++)
+            if (strcmp(names[u], names[t]) == 0) {
+                snprintf(err, errlen, "track %s given twice", names[t]);
+                rc = -1;
+            }
+        if (rc == 0) {
+            rc = load_marvel(b, names[t], an + t, dt + t, err, errlen);
+            if (rc == 1) rc = load_dazzler(b, names[t], an + t, dt + t, err, errlen);
+        }
+        if (rc == 0 && check_points(b, an[t], dt[t])) {
+            snprintf(err, errlen, "track %s: points not in increasing pairs within the reads", names[t]);
+            free(an[t]); free(dt[t]);
+            rc = -1;
+        }
+        if (rc) break;
+    }
+    if (rc) {
+        for (int u = 0; u < t; u++) { free(an[u]); free(dt[u]); }
+        free(an); free(dt);
+        return -1;
+    }
+    if (n == 1) {
+        b->manno = an[0];
+        b->mdata = dt[0];
+    } else if (n > 1) {
+        /* union of the tracks: a sweep over every read's points in increasing
+         * order, each toggling its own track in or out; a point is kept where
+         * the number of tracks inside goes from or to 0.  Points at equal
+         * positions may come in any order here and in the reference's heap:
+         * the kept lists then differ only by empty gaps, which hold no k-mer. */
+        int64_t cap = 0;
+        for (t = 0; t < n; t++) cap += an[t][b->nreads];
+        b->manno = (int64_t *)xrealloc(NULL, sizeof(int64_t) * (b->nreads + 1));
+        b->mdata = (int32_t *)xrealloc(NULL, sizeof(int32_t) * (cap + 1));
+        int64_t *cur = (int64_t *)xrealloc(NULL, sizeof(int64_t) * n);
+        int *in = (int *)xrealloc(NULL, sizeof(int) * n);
+        int64_t o = 0;
+        for (int r = 0; r < b->nreads; r++) {
+            int level = 0;
+            b->manno[r] = o;
+            for (t = 0; t < n; t++) { cur[t] = an[t][r]; in[t] = 0; }
+            for (;;) {
+                int best = -1;
+                for (t = 0; t < n; t++)
+                    if (cur[t] < an[t][r + 1] && (best < 0 || dt[t][cur[t]] < dt[best][cur[best]])) best = t;
+                if (best < 0) break;
+                int x = dt[best][cur[best]++];
+                in[best] ^= 1;
+                if (in[best]) {
+                    if (level++ == 0) b->mdata[o++] = x;
+                } else if (--level == 0) {
+                    b->mdata[o++] = x;
+                }
+            }
+        }
+        b->manno[b->nreads] = o;
+        free(cur);
+        free(in);
+        for (t = 0; t < n; t++) { free(an[t]); free(dt[t]); }
+    }
+    free(an);
+    free(dt);
+    return 0;
 }
 
 /* ======================================================================== */
@@ -1030,6 +1278,27 @@ static kmer_t *sort_kmers(const or_block *b, int K, int suppress, int64_t *nk_ou
         const char *s = b->bases + b->boff[r];
         uint64_t c = 0;
         int p = 0, x;
+        if (b->manno) {
+            /* masked: only k-mers wholly inside a gap between masked
+             * intervals, [0, pt0), [pt1, pt2), ..., [ptlast, rlen)
+             * (dalign/filter.c:416-452) */
+            for (int64_t a = b->manno[r]; a <= b->manno[r + 1]; a += 2) {
+                int q;
+                p = (a == b->manno[r]) ? 0 : b->mdata[a - 1];
+                q = (a == b->manno[r + 1]) ? b->rlen[r] : b->mdata[a];
+                if (p + K > q) continue;
+                c = 0;
+                for (x = 1; x < K; x++) c = (c << 2) | s[p++];
+                while (p < q) {
+                    c = ((c << 2) | s[p]) & kmask;
+                    list[m].code = c;
+                    list[m].read = r;
+                    list[m].rpos = p++;
+                    m++;
+                }
+            }
+            continue;
+        }
         for (x = 1; x < K; x++) c = (c << 2) | s[p++];
         while ((x = s[p]) != 4) {
             c = ((c << 2) | x) & kmask;

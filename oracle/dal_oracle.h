@@ -67,9 +67,19 @@ typedef struct {
     int64_t *boff;      /* [nreads+1] offsets of each read in bases */
     char *bases;        /* bases[-1] == 4, read i at bases+boff[i], 4 after each */
     char *alloc;
+    int ureads;         /* reads of the whole DB */
+    char *stem;         /* "DIR/.ROOT", the prefix of the DB's hidden files */
+    /* soft mask (daligner -m): read i's points at mdata[manno[i] .. manno[i+1]),
+     * pairs (begin, end) of masked intervals; NULL when unmasked */
+    int64_t *manno;
+    int32_t *mdata;
 } or_block;
 
 int or_block_load(or_block *b, const char *path);   /* "DIR/ROOT.<part>" or "ROOT" */
+/* Loads the -m tracks names[0..n) of the block (Dazzler .anno/.data or MARVEL
+ * .a2/.d2) and, for n > 1, merges them into one (dalign/daligner.c:373-431).
+ * Returns 0, or -1 with a message in err. */
+int or_block_load_masks(or_block *b, char *const *names, int n, char *err, int errlen);
 void or_block_free(or_block *b);
 /* reverse-complemented copy (dalign/daligner.c:456-573, inplace == 0) */
 void or_block_complement(const or_block *src, or_block *dst);

@@ -53,6 +53,8 @@ static void subdir(int run, int id, char *out, size_t n)
 int main(int argc, char **argv)
 {
     or_opts o;
+    char **masks = NULL;
+    int nmask = 0;
     int c, l = 1000;
     or_opts_default(&o);
     o.mem_limit = (uint64_t)sysconf(_SC_PHYS_PAGES) * (uint64_t)sysconf(_SC_PAGESIZE);
@@ -78,6 +80,10 @@ int main(int argc, char **argv)
         case 'l': l = atoi(optarg); break;
         case 's': o.tspace = atoi(optarg); break;
         case 'r': o.run_id = atoi(optarg); break;
+        case 'm':
+            masks = (char **)realloc(masks, sizeof(char *) * (nmask + 1));
+            masks[nmask++] = optarg;
+            break;
         case 'M': o.mem_limit = (uint64_t)atoi(optarg) * 0x40000000ULL; break;
         case 'j': {
             int t = atoi(optarg), e = 0;
@@ -102,8 +108,13 @@ int main(int argc, char **argv)
     }
     const char *afile = argv[optind];
     or_block A;
+    char err[512];
     if (or_block_load(&A, afile)) {
         fprintf(stderr, "cannot open %s\n", afile);
+        return 1;
+    }
+    if (nmask && or_block_load_masks(&A, masks, nmask, err, sizeof err)) {
+        printf("[ERROR] - Unable to load track: %s\n", err);
         return 1;
     }
     for (int i = 0; i < A.nreads; i++)
@@ -133,6 +144,10 @@ int main(int argc, char **argv)
         if (!self) {
             if (or_block_load(&B, bfile)) {
                 fprintf(stderr, "cannot open %s\n", bfile);
+                return 1;
+            }
+            if (nmask && or_block_load_masks(&B, masks, nmask, err, sizeof err)) {
+                printf("[ERROR] - Unable to load track: %s\n", err);
                 return 1;
             }
             if (strcmp(aname, bname) != 0) o.symmetric = 0;

@@ -41,12 +41,64 @@ CASES = {
     # pairs in the outermost diagonal bands (the band cell shared by lastp[maxdiag] and lasta[mindiag-1])
     "edge_j1": ("edges", 400, ["-j1", "-M16"], ["E.1", "E.1"]),
     "edge_j4": ("edges", 400, ["-j4", "-M16"], ["E.1", "E.1"]),
+    # soft masks (-m): the reference DBdust track of the tandem-repeat reads,
+    # random interval tracks merged with it, a MARVEL-format track, and
+    # per-block loading (a block's own track file beside the whole-DB one)
+    "rep_mdust": ("small_rep", 400, ["-mdust", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "rep_mdust_mrnd": ("small_rep", 400, ["-mdust", "-mrnd", "-j4", "-M16"], ["C1.1", "C1.1"]),
+    "rep_mm2": ("small_rep", 400, ["-mm2", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "rep_mask_blocks": ("small_rep", 1, ["-mdust", "-mrnd", "-mpt", "-j8", "-M16"], ["C1.2", "C1.1", "C1.3"]),
     # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)
     "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
 }
 LARGE = {"c2_j8"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
 
+# -m tracks a case's DB carries: name -> (layout, source).  "dust" is the
+# reference DBdust output for the reads (a fixture); "rnd<seed>" random
+# intervals; "part<seed>" writes random intervals for block 2 in its own
+# file (.ROOT.2.NAME.*) and different ones in the whole-DB file.
+MASKS = {
+    "rep_mdust": {"dust": ("dazzler", "dust")},
+    "rep_mdust_mrnd": {"dust": ("dazzler", "dust"), "rnd": ("dazzler", "rnd5")},
+    "rep_mm2": {"m2": ("marvel", "rnd6")},
+    "rep_mask_blocks": {"dust": ("dazzler", "dust"), "rnd": ("dazzler", "rnd7"), "pt": ("dazzler", "part8")},
+}
+MASK_FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mask")
+
 _reads = {}
+
+
+def random_points(lens, seed: int):
+    """Per read a point list of masked intervals: none (10 %), the whole read
+    (3 %), or up to 6 intervals of mean length 150 starting anywhere; they
+    may touch, be empty, or start at 0 / end at the read's end."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for L in lens:
+        u = rng.random()
+        if u < 0.10:
+            out.append([])
+            continue
+        if u < 0.13:
+            out.append([0, int(L)])
+            continue
+        n = int(rng.integers(1, 7))
+        starts = np.sort(rng.integers(0, L + 1, size=n))
+        if rng.random() < 0.2:
+            starts[0] = 0
+        pts, prev = [], 0
+        for b in starts:
+            b = max(int(b), prev)
+            e = min(int(L), b + int(rng.geometric(1 / 150.0)) - 1)
+            if rng.random() < 0.1:
+                e = b   # empty interval
+            pts += [b, e]
+            prev = e
+        if rng.random() < 0.2:
+            pts[-1] = int(L)
+        out.append(pts)
+    return out
 
 
 def reads_of(data: str):
@@ -58,7 +110,23 @@ def reads_of(data: str):
 def make_db(case: str, d: str) -> None:
     data, split, _, blocks = CASES[case]
     os.makedirs(d, exist_ok=True)
-    dbio.write_db(os.path.join(d, blocks[0].split(".")[0]), reads_of(data), size_mbp=split)
+    root = os.path.join(d, blocks[0].split(".")[0])
+    reads = reads_of(data)
+    dbio.write_db(root, reads, size_mbp=split)
+    lens = [len(r) for r in reads]
+    for name, (fmt, src) in MASKS.get(case, {}).items():
+        if src == "dust":
+            for ext in ("anno", "data"):
+                with open(os.path.join(MASK_FIXTURES, f"{data}.dust.{ext}"), "rb") as f, \
+                        open(f"{os.path.join(d, '.' + os.path.basename(root))}.{name}.{ext}", "wb") as g:
+                    g.write(f.read())
+        elif src.startswith("rnd"):
+            dbio.write_track(root, name, random_points(lens, int(src[3:])), fmt=fmt)
+        else:   # part<seed>: block 2's own file plus a different whole-DB track
+            seed = int(src[4:])
+            dbio.write_track(root, name, random_points(lens, seed), fmt=fmt)
+            first = dbio.block_starts(lens, split)
+            dbio.write_track(root, name, random_points(lens[first[1]:first[2]], seed + 1), fmt=fmt, block=2)
 
 
 def run(exe: str, case: str, d: str, timeout: float = 600) -> dict:

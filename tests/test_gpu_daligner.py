@@ -54,7 +54,7 @@ def test_gpu_daligner_matches_reference(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["c1_j8", "rep_j8", "c2_j8"])
+@pytest.mark.parametrize("case", ["c1_j8", "rep_j8", "rep_mdust_mrnd", "c2_j8"])
 def test_gpu_daligner_in_process(case, tmp_path):
     """Same comparison through the library loaded in this process (Python
     mirror of Set_Filter_Params / Sort_Kmers / Match_Filter /
@@ -64,6 +64,7 @@ def test_gpu_daligner_in_process(case, tmp_path):
     _, _, flags, blocks = las_cases.CASES[case]
     assert blocks[0] == blocks[1], "self comparisons only"
     opts = {f[1]: f[2:] for f in flags}
+    masks = [f[2:] for f in flags if f.startswith("-m")]
     las_cases.make_db(case, str(tmp_path))
     dal = Daligner()
     nthreads = int(opts.get("j", "4"))
@@ -71,6 +72,8 @@ def test_gpu_daligner_in_process(case, tmp_path):
     dal.set_filter_params(int(opts.get("k", "14")), int(opts.get("w", "6")), int(opts.get("t", "0")),
                           int(opts.get("h", "35")))
     A = dal.open_block(str(tmp_path / blocks[0]))
+    if masks:
+        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(float(opts.get("e", ".70")), int(opts.get("s", "100")), list(A.freq), nthreads=nthreads)
     dev = dal.block_device(A)
     cwd = os.getcwd()
@@ -88,10 +91,12 @@ INTEG = os.path.join(ROOT, "oracle", "_ref", "daligner_on_libmarvl")
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(INTEG), reason="integration binary not built (make -C oracle integ)")
-@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "c2_j8"])
+@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "rep_mdust_mrnd", "rep_mask_blocks", "c2_j8"])
 def test_reference_driver_on_library(case):
     """The reference's own daligner.c and DB code, linked against
-    libmarvl_align instead of its filter.c / align.c, writes the golden .las."""
+    libmarvl_align instead of its filter.c / align.c, writes the golden .las.
+    With -m the reference's read_DB / complement_DB build the mask tracks and
+    the library's Sort_Kmers reads them from HITS_DB.tracks."""
     print(f"[reference driver on libmarvl_align] case {case}", flush=True)
     with tempfile.TemporaryDirectory() as d:
         las_cases.make_db(case, d)
@@ -117,6 +122,8 @@ def test_band_edge_cell_rerun(tmp_path):
     dal.configure(nthreads=1, mem_limit_gb=16, minover=2000)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(str(tmp_path / blocks[0]))
+    if masks:
+        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=1)
     cwd = os.getcwd()
     os.chdir(tmp_path)
@@ -144,6 +151,8 @@ def test_output_arena_overflow_reruns(tmp_path, monkeypatch):
     dal.configure(nthreads=8, mem_limit_gb=16, minover=2000)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(str(tmp_path / blocks[0]))
+    if masks:
+        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=8)
     cwd = os.getcwd()
     os.chdir(tmp_path)
@@ -174,6 +183,8 @@ def test_every_report_window_matches_reference(tmp_path, monkeypatch, level):
     dal.configure(nthreads=8, mem_limit_gb=16, minover=2000)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(str(tmp_path / blocks[0]))
+    if masks:
+        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=8)
     cwd = os.getcwd()
     os.chdir(tmp_path)

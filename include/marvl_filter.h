@@ -189,6 +189,16 @@ void mi_complement_header(const HITS_DB *src, HITS_DB *dst);
  * beyond 1 twice, reading outside the block's slice; this loader takes the
  * block's own slice.  Returns 0 or MI_ERR_ARG (message in mi_last_error). */
 int mi_load_mask_tracks(HITS_DB *db, const char *const *names, int n);
+/* LAsort on the GPU: the sort and output loop of dalign/LAsort.c:183-235
+ * (records ordered by (aread, bread, abpos), SORT_OVL at :60-86; equal keys
+ * keep their input order, as under glibc's merge-sort qsort).  `body` is
+ * the `size` bytes after a .las header holding `novl` records; tspace <= 125
+ * means 1-byte trace values, else 2.  The sorted records go to `out` (at
+ * least *out_size bytes: the records' bytes, without anything after the
+ * last one).  Returns 0, MI_ERR_ARG when the records do not fit in `size`,
+ * or MI_ERR_DEVICE. */
+int mi_las_sort(const void *body, long long size, long long novl, int tspace, void *out, long long *out_size);
+
 /* Frees the tracks this library allocated for db (mi_load_mask_tracks,
  * mi_complement_header) and clears db->tracks; mi_close_block calls it. */
 void mi_release_tracks(HITS_DB *db);

@@ -56,6 +56,15 @@ def build(verbose: bool = False) -> str:
           os.path.join(CSRC, CLI_SOURCE), "-o", exe + ".tmp", "-L", HERE, "-lmarvl_align",
           "-Wl,-rpath,$ORIGIN/.."])
     os.replace(exe + ".tmp", exe)
+    # the .las utilities: LAsort over the library's GPU sort, LAcat (host only)
+    for name, src, lib in (("LAsort", "lasort_main.cpp", True), ("LAcat", "lacat_main.cpp", False)):
+        exe = os.path.join(bindir, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, src),
+               "-o", exe + ".tmp"]
+        if lib:
+            cmd += ["-L", HERE, "-lmarvl_align", "-Wl,-rpath,$ORIGIN/.."]
+        _run(cmd)
+        os.replace(exe + ".tmp", exe)
     # the block-pair plan emitter (host only)
     hpc = os.path.join(bindir, "HPCdaligner")
     _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "hpcdaligner_main.cpp"), "-o", hpc + ".tmp"])

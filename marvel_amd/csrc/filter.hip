@@ -1398,3 +1398,145 @@ void Match_Filter(char* aname, HITS_DB* ablock, char* bname, HITS_DB* bblock, vo
 void mi_filter_stats(MiFilterStats* out) { *out = g_stats; }
 
 }  // extern "C"
+
+// ---- LAsort (dalign/LAsort.c) ------------------------------------------------------
+// The record order of LAsort: (aread, bread, abpos), equal keys in input
+// order (SORT_OVL, dalign/LAsort.c:60-86, under glibc's qsort, a stable merge
+// sort).  The host walks the records once (their offsets depend on every
+// earlier trace length) and builds the keys; the GPU sorts them with the
+// stable radix sort and gathers the records into the output.
+
+namespace mi {
+
+// stage-2 keys in stage-1 order: out[j] = {ab[in[j].idx], in[j].idx}
+__global__ void k_las_rekey(const RecKey* __restrict__ in, const unsigned long long* __restrict__ ab, long long n,
+                            RecKey* __restrict__ out)
+{
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const long long i = in[j].idx;
+    out[j] = RecKey{ab[i], i};
+}
+
+__global__ void k_las_sizes(const RecKey* __restrict__ sorted, const long long* __restrict__ size, long long n,
+                            u64* __restrict__ out)
+{
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    out[j] = (u64)size[sorted[j].idx];
+}
+
+// one wave per output record: its bytes from the input body (records start at
+// any byte offset: 1-byte trace values make odd lengths)
+__global__ __launch_bounds__(256) void k_las_gather(const uint8_t* __restrict__ body, const long long* __restrict__ off,
+                                                    const long long* __restrict__ size, const RecKey* __restrict__ sorted,
+                                                    const u64* __restrict__ dpos, long long n, uint8_t* __restrict__ out)
+{
+    const long long j = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= n) return;
+    const int l = threadIdx.x & 63;
+    const long long i = sorted[j].idx;
+    const uint8_t* src = body + off[i];
+    uint8_t* dst = out + dpos[j];
+    const long long m = size[i];
+    for (long long b = l; b < m; b += 64) dst[b] = src[b];
+}
+
+}  // namespace mi
+
+extern "C" int mi_las_sort(const void* body, long long size, long long novl, int tspace, void* out, long long* out_size)
+{
+    try {
+        if (novl < 0 || size < 0 || (novl > 0 && (!body || !out)))
+            throw Fail("mi_las_sort: bad arguments", MI_ERR_ARG);
+        const int tb = tspace <= 125 ? 1 : 2;   // TRACE_XOVR (dalign/align.h)
+        const uint8_t* b = (const uint8_t*)body;
+        std::vector<long long> off(novl), sz(novl);
+        std::vector<int> ar(novl), br(novl), ap(novl);
+        long long o = 0;
+        int amin = INT32_MAX, amax = INT32_MIN, bmin = INT32_MAX, bmax = INT32_MIN, pmin = INT32_MAX, pmax = INT32_MIN;
+        for (long long i = 0; i < novl; i++) {
+            if (o + 40 > size) throw Fail("mi_las_sort: records run past the end of the file", MI_ERR_ARG);
+            int tlen;
+            memcpy(&tlen, b + o, 4);
+            memcpy(&ap[i], b + o + 8, 4);
+            memcpy(&ar[i], b + o + 28, 4);
+            memcpy(&br[i], b + o + 32, 4);
+            const long long n = 40 + (long long)tlen * tb;
+            if (tlen < 0 || o + n > size) throw Fail("mi_las_sort: records run past the end of the file", MI_ERR_ARG);
+            off[i] = o;
+            sz[i] = n;
+            o += n;
+            amin = std::min(amin, ar[i]);
+            amax = std::max(amax, ar[i]);
+            bmin = std::min(bmin, br[i]);
+            bmax = std::max(bmax, br[i]);
+            pmin = std::min(pmin, ap[i]);
+            pmax = std::max(pmax, ap[i]);
+        }
+        if (out_size) *out_size = o;
+        if (novl == 0) return 0;
+        if (novl == 1) {
+            memcpy(out, body, o);
+            return 0;
+        }
+        const int ra = bits_for((long long)amax - amin), rb = bits_for((long long)bmax - bmin),
+                  pb = bits_for((long long)pmax - pmin);
+        const bool one = ra + rb + pb <= 64;
+        std::vector<RecKey> keys(novl);
+        std::vector<unsigned long long> ab;
+        for (long long i = 0; i < novl; i++) {
+            const unsigned long long a = (unsigned long long)((long long)ar[i] - amin),
+                                     c = (unsigned long long)((long long)br[i] - bmin),
+                                     p = (unsigned long long)((long long)ap[i] - pmin);
+            keys[i] = RecKey{one ? (a << (rb + pb)) | (c << pb) | p : p, i};
+        }
+        if (!one) {
+            ab.resize(novl);
+            for (long long i = 0; i < novl; i++)
+                ab[i] = ((unsigned long long)((long long)ar[i] - amin) << rb) | (unsigned long long)((long long)br[i] - bmin);
+        }
+        hipStream_t st = fstream();
+        DBuf<uint8_t> dbody, dout;
+        DBuf<long long> doff, dsz;
+        DBuf<RecKey> k1, k2;
+        DBuf<unsigned long long> dab;
+        DBuf<u64> s1, pos;
+        dbody.ensure(o);
+        dout.ensure(o);
+        doff.ensure(novl);
+        dsz.ensure(novl);
+        k1.ensure(novl);
+        k2.ensure(novl);
+        s1.ensure(novl);
+        pos.ensure(novl + 1);
+        HC(hipMemcpyAsync(dbody.p, body, o, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(doff.p, off.data(), sizeof(long long) * novl, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(dsz.p, sz.data(), sizeof(long long) * novl, hipMemcpyHostToDevice, st));
+        HC(hipMemcpyAsync(k1.p, keys.data(), sizeof(RecKey) * novl, hipMemcpyHostToDevice, st));
+        RecKey* sorted = radix_sort(k1.p, k2.p, novl, 0, one ? ra + rb + pb : pb, RecKeyOf{});
+        if (!one) {   // then (aread, bread), stable over the abpos order
+            dab.ensure(novl);
+            HC(hipMemcpyAsync(dab.p, ab.data(), sizeof(unsigned long long) * novl, hipMemcpyHostToDevice, st));
+            RecKey* other = sorted == k1.p ? k2.p : k1.p;
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_las_rekey), dim3(nblk(novl)), dim3(256), 0, st, sorted, dab.p, novl, other);
+            HC(hipGetLastError());
+            sorted = radix_sort(other, sorted, novl, 0, ra + rb, RecKeyOf{});
+        }
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_las_sizes), dim3(nblk(novl)), dim3(256), 0, st, sorted, dsz.p, novl, s1.p);
+        HC(hipGetLastError());
+        g_scan.run(s1.p, pos.p, novl);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_las_gather), dim3((unsigned)((novl + 3) / 4)), dim3(256), 0, st, dbody.p, doff.p,
+                           dsz.p, sorted, pos.p, novl, dout.p);
+        HC(hipGetLastError());
+        HC(hipMemcpyAsync(out, dout.p, o, hipMemcpyDeviceToHost, st));
+        HC(hipStreamSynchronize(st));
+        return 0;
+    } catch (const Fail& e) {
+        mi_set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        mi_set_error("mi_las_sort: host out of memory");
+        return MI_ERR_DEVICE;
+    }
+}

@@ -211,6 +211,17 @@ def read_las(path: str):
     return tspace, recs, traces
 
 
+def write_las(path: str, tspace: int, recs, traces) -> None:
+    """Writes a .las file: header (novl, tspace), then each record's 40 bytes
+    and its trace bytes (1 byte per value for tspace <= 125, else 2)."""
+    recs = np.asarray(recs, dtype=LAS_REC)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<qi", len(recs), tspace))
+        for r, t in zip(recs, traces):
+            f.write(r.tobytes())
+            f.write(bytes(t))
+
+
 def first_difference(path_a: str, path_b: str):
     """None when the files are byte-identical, else a description of the first differing record."""
     with open(path_a, "rb") as f:

@@ -122,8 +122,6 @@ def test_band_edge_cell_rerun(tmp_path):
     dal.configure(nthreads=1, mem_limit_gb=16, minover=2000)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(str(tmp_path / blocks[0]))
-    if masks:
-        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=1)
     cwd = os.getcwd()
     os.chdir(tmp_path)
@@ -151,8 +149,6 @@ def test_output_arena_overflow_reruns(tmp_path, monkeypatch):
     dal.configure(nthreads=8, mem_limit_gb=16, minover=2000)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(str(tmp_path / blocks[0]))
-    if masks:
-        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=8)
     cwd = os.getcwd()
     os.chdir(tmp_path)
@@ -183,8 +179,6 @@ def test_every_report_window_matches_reference(tmp_path, monkeypatch, level):
     dal.configure(nthreads=8, mem_limit_gb=16, minover=2000)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(str(tmp_path / blocks[0]))
-    if masks:
-        dal.load_mask_tracks(A, masks)   # the device block keeps the mask
     spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=8)
     cwd = os.getcwd()
     os.chdir(tmp_path)

@@ -198,6 +198,14 @@ int mi_load_mask_tracks(HITS_DB *db, const char *const *names, int n);
  * last one).  Returns 0, MI_ERR_ARG when the records do not fit in `size`,
  * or MI_ERR_DEVICE. */
 int mi_las_sort(const void *body, long long size, long long novl, int tspace, void *out, long long *out_size);
+/* LAmerge on the GPU: the order of utils/LAmerge.c over the records of all
+ * its inputs, concatenated in input order -- (aread, bread, COMP, abpos)
+ * (SORT_OVL, utils/LAmergeUtils.c:50-80), equal keys in input order.  That
+ * is the reference's output for sorted inputs whose equal keys lie in one
+ * file (its heap merge, :97-141; block merges, where each file holds other
+ * B reads) and for -s (stable qsort of the concatenation).  Arguments as
+ * mi_las_sort. */
+int mi_las_merge(const void *body, long long size, long long novl, int tspace, void *out, long long *out_size);
 
 /* Frees the tracks this library allocated for db (mi_load_mask_tracks,
  * mi_complement_header) and clears db->tracks; mi_close_block calls it. */

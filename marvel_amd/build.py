@@ -56,8 +56,9 @@ def build(verbose: bool = False) -> str:
           os.path.join(CSRC, CLI_SOURCE), "-o", exe + ".tmp", "-L", HERE, "-lmarvl_align",
           "-Wl,-rpath,$ORIGIN/.."])
     os.replace(exe + ".tmp", exe)
-    # the .las utilities: LAsort over the library's GPU sort, LAcat (host only)
-    for name, src, lib in (("LAsort", "lasort_main.cpp", True), ("LAcat", "lacat_main.cpp", False)):
+    # the .las utilities: LAsort and LAmerge over the library's GPU sort, LAcat (host only)
+    for name, src, lib in (("LAsort", "lasort_main.cpp", True), ("LAmerge", "lamerge_main.cpp", True),
+                           ("LAcat", "lacat_main.cpp", False)):
         exe = os.path.join(bindir, name)
         cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, src),
                "-o", exe + ".tmp"]

@@ -1444,7 +1444,10 @@ __global__ __launch_bounds__(256) void k_las_gather(const uint8_t* __restrict__ 
 
 }  // namespace mi
 
-extern "C" int mi_las_sort(const void* body, long long size, long long novl, int tspace, void* out, long long* out_size)
+// with_comp: LAmerge's order (aread, bread, COMP, abpos; SORT_OVL,
+// utils/LAmergeUtils.c:50-80), else LAsort's (aread, bread, abpos)
+static int las_order(const void* body, long long size, long long novl, int tspace, bool with_comp, void* out,
+                     long long* out_size)
 {
     try {
         if (novl < 0 || size < 0 || (novl > 0 && (!body || !out)))
@@ -1453,6 +1456,7 @@ extern "C" int mi_las_sort(const void* body, long long size, long long novl, int
         const uint8_t* b = (const uint8_t*)body;
         std::vector<long long> off(novl), sz(novl);
         std::vector<int> ar(novl), br(novl), ap(novl);
+        std::vector<unsigned char> cp(novl);
         long long o = 0;
         int amin = INT32_MAX, amax = INT32_MIN, bmin = INT32_MAX, bmax = INT32_MIN, pmin = INT32_MAX, pmax = INT32_MIN;
         for (long long i = 0; i < novl; i++) {
@@ -1462,6 +1466,7 @@ extern "C" int mi_las_sort(const void* body, long long size, long long novl, int
             memcpy(&ap[i], b + o + 8, 4);
             memcpy(&ar[i], b + o + 28, 4);
             memcpy(&br[i], b + o + 32, 4);
+            cp[i] = with_comp ? (b[o + 24] & 1) : 0;   // COMP flag, bit 0 of flags (lib/oflags.h)
             const long long n = 40 + (long long)tlen * tb;
             if (tlen < 0 || o + n > size) throw Fail("mi_las_sort: records run past the end of the file", MI_ERR_ARG);
             off[i] = o;
@@ -1481,14 +1486,15 @@ extern "C" int mi_las_sort(const void* body, long long size, long long novl, int
             return 0;
         }
         const int ra = bits_for((long long)amax - amin), rb = bits_for((long long)bmax - bmin),
-                  pb = bits_for((long long)pmax - pmin);
+                  pb = bits_for((long long)pmax - pmin) + (with_comp ? 1 : 0);   // COMP above abpos
         const bool one = ra + rb + pb <= 64;
         std::vector<RecKey> keys(novl);
         std::vector<unsigned long long> ab;
         for (long long i = 0; i < novl; i++) {
             const unsigned long long a = (unsigned long long)((long long)ar[i] - amin),
                                      c = (unsigned long long)((long long)br[i] - bmin),
-                                     p = (unsigned long long)((long long)ap[i] - pmin);
+                                     p = ((unsigned long long)cp[i] << (pb - 1)) |
+                                         (unsigned long long)((long long)ap[i] - pmin);
             keys[i] = RecKey{one ? (a << (rb + pb)) | (c << pb) | p : p, i};
         }
         if (!one) {
@@ -1539,4 +1545,14 @@ extern "C" int mi_las_sort(const void* body, long long size, long long novl, int
         mi_set_error("mi_las_sort: host out of memory");
         return MI_ERR_DEVICE;
     }
+}
+
+extern "C" int mi_las_sort(const void* body, long long size, long long novl, int tspace, void* out, long long* out_size)
+{
+    return las_order(body, size, novl, tspace, false, out, out_size);
+}
+
+extern "C" int mi_las_merge(const void* body, long long size, long long novl, int tspace, void* out, long long* out_size)
+{
+    return las_order(body, size, novl, tspace, true, out, out_size);
 }

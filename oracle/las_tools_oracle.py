@@ -5,6 +5,10 @@ the product path).
   las_sort  dalign/LAsort.c:60-86, :183-235 -- records ordered by (aread, bread,
             abpos); qsort is glibc's merge sort, so equal keys keep their input
             order (a stable sort)
+  las_merge utils/LAmerge.c:306-506 (heap merge), :17-133 (-s), SORT_OVL at
+            utils/LAmergeUtils.c:50-80 -- the records of all inputs in
+            (aread, bread, COMP, abpos) order; for sorted inputs whose equal
+            keys lie in one file, and for -s, equal keys keep input order
   las_cat   dalign/LAcat.c:66-190 -- ROOT.1.las, ROOT.2.las, ... up to the first
             missing file; header (sum of counts, first file's tspace); the
             records of each file in order
@@ -61,3 +65,24 @@ def las_cat(root: str) -> bytes:
         novl += n
         body += recs
     return struct.pack("<qi", novl, tspace) + b"".join(body)
+
+
+def las_merge(raws) -> bytes:
+    """LAmerge's output for the .las images `raws` (in input order)."""
+    if len(raws) == 1:
+        return raws[0]
+    novl, tspace, recs = 0, None, []
+    for raw in raws:
+        n, ts, r = split_records(raw)
+        if tspace is None:
+            tspace = ts
+        elif ts != tspace:
+            raise ValueError(f"PT-point spacing conflict ({tspace} vs {ts})")
+        novl += n
+        recs += r
+    hdr = struct.pack("<qi", novl, tspace)
+    if not recs:
+        return hdr
+    h = np.frombuffer(b"".join(r[:40] for r in recs), dtype=REC)
+    order = np.lexsort((h["abpos"], h["flags"] & 1, h["bread"], h["aread"]))
+    return hdr + b"".join(recs[i] for i in order)

@@ -1,5 +1,5 @@
 """Writes tests/golden/la_tools/goldens.json: SHA-256 of what the reference
-LAsort / LAcat (oracle/_ref, compiled from /root/reference by
+LAsort / LAcat / LAmerge (oracle/_ref, compiled from /root/reference by
 `make -C oracle ref`) write for the inputs of tests/la_tool_cases.py.
 
     python tests/golden/make_la_tool_goldens.py

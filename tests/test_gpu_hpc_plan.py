@@ -2,7 +2,9 @@
 small read set split into 3 blocks, the GPU daligner runs every daligner
 line of the plan, and all nine .las files equal the reference daligner's run
 of the reference plan (tests/golden/hpc/plan_run.json, from
-tests/golden/make_plan_goldens.py).  The reference plan passes -M16 on as
+tests/golden/make_plan_goldens.py); then the GPU LAmerge runs the plan's
+merge lines (a directory of block-pair files each) and the three block
+files equal the reference LAmerge's.  The reference plan passes -M16 on as
 -s16, so the run also covers 16-base trace spacing."""
 
 import hashlib
@@ -21,6 +23,7 @@ import make_plan_goldens as mp  # noqa: E402
 GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "hpc", "plan_run.json")))
 HPC = os.path.join(ROOT, "marvel_amd", "bin", "HPCdaligner")
 DAL = os.path.join(ROOT, "marvel_amd", "bin", "daligner")
+MRG = os.path.join(ROOT, "marvel_amd", "bin", "LAmerge")
 
 
 @pytest.mark.gpu
@@ -41,3 +44,8 @@ def test_plan_run_matches_reference(tmp_path):
                 p = os.path.join(dp, f)
                 got[os.path.relpath(p, d)] = hashlib.sha256(open(p, "rb").read()).hexdigest()
     assert got == GOLD["las"]
+    mlines = mp.merge_lines(HPC, d)
+    assert mlines == GOLD["merge_lines"]
+    for args in mlines:
+        print("[hpc plan] LAmerge " + " ".join(args), flush=True)
+    assert mp.run_merges(MRG, mlines, d) == GOLD["merged"]

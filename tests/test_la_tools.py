@@ -1,6 +1,6 @@
 """The .las utilities after the path against the compiled reference's
 outputs (tests/golden/la_tools): the oracle restatement (CPU), the host
-LAcat (CPU), and the GPU LAsort (gpu marker)."""
+LAcat (CPU), and the GPU LAsort / LAmerge (gpu marker)."""
 
 import hashlib
 import json
@@ -30,6 +30,10 @@ def test_oracle_matches_reference(case, tmp_path):
     la_tool_cases.make_inputs(case, str(tmp_path))
     if tool == "LAcat":
         got = {"stdout": _sha(las_tools_oracle.las_cat(str(tmp_path / args[0])))}
+    elif tool == "LAmerge":
+        names = [f[0] for f in files]   # inputs in the order the cases name them (bread ranges differ per file)
+        got = {la_tool_cases.MERGE_OUT[case]: _sha(las_tools_oracle.las_merge(
+            [open(tmp_path / n, "rb").read() for n in names]))}
     else:
         got = {}
         for a in args:
@@ -61,6 +65,16 @@ def test_gpu_lasort_matches_reference(case, tmp_path):
     exe = os.path.join(BIN, "LAsort")
     assert os.path.exists(exe), "LAsort not built (python -m marvel_amd.build)"
     print(f"[gpu LAsort] case {case}", flush=True)
+    la_tool_cases.make_inputs(case, str(tmp_path))
+    assert la_tool_cases.run(exe, case, str(tmp_path)) == GOLD[case]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(c for c, v in la_tool_cases.CASES.items() if v[0] == "LAmerge"))
+def test_gpu_lamerge_matches_reference(case, tmp_path):
+    exe = os.path.join(BIN, "LAmerge")
+    assert os.path.exists(exe), "LAmerge not built (python -m marvel_amd.build)"
+    print(f"[gpu LAmerge] case {case}", flush=True)
     la_tool_cases.make_inputs(case, str(tmp_path))
     assert la_tool_cases.run(exe, case, str(tmp_path)) == GOLD[case]
 

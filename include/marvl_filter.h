@@ -207,6 +207,15 @@ int mi_las_sort(const void *body, long long size, long long novl, int tspace, vo
  * mi_las_sort. */
 int mi_las_merge(const void *body, long long size, long long novl, int tspace, void *out, long long *out_size);
 
+/* DBdust on the GPU: the SDUST scan of db/DBdust.c:236-470 over every read
+ * of `db` (bases 0..3 at db->bases + reads[i].boff, as Read_All_Sequences
+ * leaves them), one lane per read.  window (-w, 1..256), threshold (-t),
+ * mlen (-m less one: intervals of at least mlen are kept), biased (-b, with
+ * db->freq).  Writes anno[0..nreads] (byte offsets, as in the .dust.anno
+ * file) and the points (pairs begin, end+1; at most the block's total read
+ * length of them).  Returns 0, MI_ERR_ARG or MI_ERR_DEVICE. */
+int mi_dust(const HITS_DB *db, int window, double threshold, int mlen, int biased, long long *anno, int *points);
+
 /* Frees the tracks this library allocated for db (mi_load_mask_tracks,
  * mi_complement_header) and clears db->tracks; mi_close_block calls it. */
 void mi_release_tracks(HITS_DB *db);

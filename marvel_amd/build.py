@@ -16,7 +16,7 @@ ARCH = os.environ.get("MI_OFFLOAD_ARCH", "gfx950")
 DEV_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
-LIB_SOURCES = ["la_batch.hip", "filter.hip", "trace_pts.hip", "scorer.hip", "selftest.hip", "spec.cpp", "align_capi.cpp", "ovl_buffer.cpp",
+LIB_SOURCES = ["la_batch.hip", "filter.hip", "trace_pts.hip", "scorer.hip", "dust.hip", "selftest.hip", "spec.cpp", "align_capi.cpp", "ovl_buffer.cpp",
                "db_block.cpp", "tracks.cpp"]
 CLI_SOURCE = "daligner_main.cpp"
 
@@ -58,6 +58,7 @@ def build(verbose: bool = False) -> str:
     os.replace(exe + ".tmp", exe)
     # the .las utilities: LAsort and LAmerge over the library's GPU sort, LAcat (host only)
     for name, src, lib in (("LAsort", "lasort_main.cpp", True), ("LAmerge", "lamerge_main.cpp", True),
+                           ("DBdust", "dbdust_main.cpp", True),
                            ("LAcat", "lacat_main.cpp", False)):
         exe = os.path.join(bindir, name)
         cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, src),

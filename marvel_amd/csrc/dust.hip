@@ -126,8 +126,14 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
                     const int d = T(c);
                     lsqr += lcnt[d * 64 + lane]++;
                     if (lsqr >= a.thresh * (j - c)) {
-                        for (; BEG(J) >= c; Lp = J, J = NX(J))
+                        for (int g = 0; BEG(J) >= c; Lp = J, J = NX(J)) {
+                            if (++g > a.window + 1) {   // a list longer than its nodes: stop the read
+                                bad = true;
+                                break;
+                            }
                             if (SC(J) > mscore) mscore = SC(J);
+                        }
+                        if (bad) break;
                         if (lsqr >= mscore * (j - c)) {
                             mscore = (1. * lsqr) / (j - c);
                             if (BEG(Lp) == c) {
@@ -188,8 +194,14 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
                     const int d = T(c);
                     lsqr += (lcnt[d * 64 + lane]++) * a.skew[d];
                     if (lsqr >= a.thresh * (j - c)) {
-                        for (; BEG(J) >= c; Lp = J, J = NX(J))
+                        for (int g = 0; BEG(J) >= c; Lp = J, J = NX(J)) {
+                            if (++g > a.window + 1) {   // a list longer than its nodes: stop the read
+                                bad = true;
+                                break;
+                            }
                             if (SC(J) > mscore) mscore = SC(J);
+                        }
+                        if (bad) break;
                         if (lsqr >= mscore * (j - c)) {
                             mscore = lsqr / (j - c);
                             if (BEG(Lp) == c) {
@@ -225,7 +237,12 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
             a.npts[r] = 0;
             continue;
         }
-        for (int J = PV(0); J != 0; J = PV(0)) retire(J);
+        for (int J = PV(0), g = 0; J != 0 && g <= a.window; J = PV(0), g++) retire(J);
+        if (PV(0) != 0) {
+            a.status[r] = 1;
+            a.npts[r] = 0;
+            continue;
+        }
         // keep intervals of at least mlen as points (b, e + 1), in place
         int n = 0;
         for (int k = 1; k < top; k += 2)

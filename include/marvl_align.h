@@ -118,10 +118,17 @@ int Symmetric(Align_Spec *espec);
 /* Write_Overlap_Buffer: replaces dalign/align.c:5815-6043.  Stable sort of
  * all buffered records by (aread, bread, COMP, abpos), then one .las file
  * (self or not symmetric) or two split at lastRead, named
- * [RRR_BBBBB/]A.a.B.b.las as the reference names them. */
+ * [RRR_BBBBB/]A.a.B.b.las as the reference names them.  The sort and the
+ * write run on a background thread, overlapping the caller's next block
+ * pair; the call returns once the previous write is done.  The next write,
+ * Free_Align_Spec, mi_flush_overlap_writes and process exit wait for it (a
+ * failed write ends the process with status 1, as in the reference). */
 void Write_Overlap_Buffer(Align_Spec *spec, int RUN_ID, char *ablock, char *bblock, int lastRead);
 /* Reset_Overlap_Buffer: replaces dalign/align.c:6045-6056. */
 void Reset_Overlap_Buffer(Align_Spec *spec);
+/* Waits for the .las files handed to the background writer: 0, or 1 when a
+ * write failed.  (No reference counterpart: its writes are synchronous.) */
+int mi_flush_overlap_writes(void);
 
 /* ---- batched entry points (no reference counterpart) ------------------- */
 

@@ -62,11 +62,17 @@ Align_Spec* New_Align_Spec(double ave_corr, int trace_space, float* freq, int nt
 void Free_Align_Spec(Align_Spec* spec)
 {
     SpecObj* s = (SpecObj*)spec;
+    mi::wait_overlap_writes();
     for (int i = 0; i < s->nthreads; i++) {
         free(s->iobuf[i].ovls);
         free(s->iobuf[i].trace);
+        if (s->spare) {
+            free(s->spare[i].ovls);
+            free(s->spare[i].trace);
+        }
     }
     free(s->iobuf);
+    free(s->spare);
     delete s;
 }
 int Trace_Spacing(Align_Spec* spec) { return ((SpecObj*)spec)->hs.tspace; }

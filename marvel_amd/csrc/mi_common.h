@@ -30,7 +30,15 @@ struct SpecObj {
     MiAlignParams prm;
     int nthreads, symmetric, only_identity, no_trace;
     Overlap_IO_Buffer* iobuf;   // [nthreads]
+    // double buffering for the background .las writer: the buffers a write
+    // is still reading become the spare set at Reset_Overlap_Buffer, and the
+    // spare set (already written) becomes the active one
+    Overlap_IO_Buffer* spare = nullptr;   // [nthreads], made at the first reset after a write
+    bool written = false;                 // the active buffers were handed to the writer
 };
+// Waits for the background .las writer; the process exits with status 1
+// when a write failed (the reference exits inside Write_Overlap_Buffer).
+void wait_overlap_writes();
 
 void spec_build(HostSpec* s, double ave_corr, int tspace, const float* freq);
 struct OutRec;

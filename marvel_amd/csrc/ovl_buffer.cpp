@@ -10,7 +10,10 @@
 #include <fcntl.h>
 #include <unistd.h>
 
+#include <condition_variable>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -155,12 +158,98 @@ bool write_las(const std::string& path, const std::vector<const Overlap*>& recs,
 
 }  // namespace
 
+namespace {
+
+// One background thread writes .las files: Write_Overlap_Buffer hands it a
+// snapshot of the buffer headers and returns; the next write,
+// Reset_Overlap_Buffer's swap, appends to buffers it is still reading,
+// Free_Align_Spec, mi_flush_overlap_writes and process exit wait for it.
+// It lives until the process ends (never destroyed, so exit cannot tear it
+// down under a running job).
+struct Writer {
+    std::mutex m;
+    std::condition_variable cv;
+    std::function<bool()> job;
+    bool busy = false, failed = false;
+    std::vector<const Overlap*> reading;   // ovls arrays the running job reads
+    Writer()
+    {
+        std::thread([this] {
+            for (;;) {
+                std::function<bool()> f;
+                {
+                    std::unique_lock<std::mutex> l(m);
+                    cv.wait(l, [this] { return job != nullptr; });
+                    f = std::move(job);
+                    job = nullptr;
+                }
+                const bool ok = f();
+                std::lock_guard<std::mutex> l(m);
+                failed = failed || !ok;
+                busy = false;
+                reading.clear();
+                cv.notify_all();
+            }
+        }).detach();
+    }
+    // waits until idle; true when every write so far succeeded
+    bool wait()
+    {
+        std::unique_lock<std::mutex> l(m);
+        cv.wait(l, [this] { return !busy; });
+        const bool ok = !failed;
+        failed = false;
+        return ok;
+    }
+    void submit(std::function<bool()> f, std::vector<const Overlap*> arrays)
+    {
+        std::lock_guard<std::mutex> l(m);
+        job = std::move(f);
+        busy = true;
+        reading = std::move(arrays);
+        cv.notify_all();
+    }
+    bool reads(const Overlap* a)
+    {
+        std::lock_guard<std::mutex> l(m);
+        return busy && std::find(reading.begin(), reading.end(), a) != reading.end();
+    }
+};
+
+Writer& writer()
+{
+    static Writer* w = [] {
+        Writer* x = new Writer;
+        atexit([] { mi::wait_overlap_writes(); });
+        return x;
+    }();
+    return *w;
+}
+
+// an append to a buffer the writer is reading first waits for the write
+void before_append(const Overlap_IO_Buffer* b)
+{
+    Writer& w = writer();
+    if (w.reads(b->ovls)) mi::wait_overlap_writes();
+}
+
+}  // namespace
+
 namespace mi {
+
+void wait_overlap_writes()
+{
+    if (!writer().wait()) {
+        fprintf(stderr, "[ERROR] - Write_Overlap_Buffer: write failed\n");
+        _exit(1);
+    }
+}
 
 int mi_append_records(Overlap_IO_Buffer* b, const OutRec* recs, long long n, const uint8_t* tr, long long trbytes,
                       int tbytes)
 {
     if (n <= 0) return 0;
+    before_append(b);
     if (b->otop + n > b->omax) {
         const long long nmax = (long long)((b->otop + n) * 1.2) + 1000;
         if (nmax > INT32_MAX) return 1;
@@ -248,6 +337,7 @@ Overlap_IO_Buffer* CreateOverlapBuffer(int nthreads, int tbytes, int no_trace)
 int AddOverlapToBuffer(Overlap_IO_Buffer* b, Overlap* ovl, int tbytes)
 {
     if (b == nullptr) return 1;
+    before_append(b);
     if (b->otop == b->omax) {
         const int nmax = (int)(b->omax * 1.2) + 1000;
         Overlap* o = (Overlap*)realloc(b->ovls, sizeof(Overlap) * nmax);
@@ -296,9 +386,15 @@ int Num_Threads(Align_Spec* espec) { return ((SpecObj*)espec)->nthreads; }
 int Only_Identity(Align_Spec* espec) { return ((SpecObj*)espec)->only_identity; }
 int Symmetric(Align_Spec* espec) { return ((SpecObj*)espec)->symmetric; }
 
-void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblock, int lastRead)
+}  // extern "C"
+
+namespace {
+
+// Sorts the records of buffers buf[0 .. nthreads) and writes the .las
+// file(s) of the block pair; false when a write failed.
+bool write_pair(const Overlap_IO_Buffer* buf, int nthreads, int only_identity, int symmetric, int tspace, int RUN_ID,
+                const std::string& ablock, const std::string& bblock, int lastRead)
 {
-    SpecObj* s = (SpecObj*)spec;
     // sort keys of all buffered records, thread buffers in order (the stable
     // order keeps that order for equal keys)
     struct Key {
@@ -315,12 +411,12 @@ void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblo
     keys.clear();
     {
         size_t n = 0;
-        for (int i = 0; i < s->nthreads; i++) n += s->iobuf[i].otop;
+        for (int i = 0; i < nthreads; i++) n += buf[i].otop;
         keys.reserve(n);
-        for (int i = 0; i < s->nthreads; i++)
-            for (int j = 0; j < s->iobuf[i].otop; j++) {
-                const Overlap* o = s->iobuf[i].ovls + j;
-                if (!s->only_identity || o->aread == o->bread)
+        for (int i = 0; i < nthreads; i++)
+            for (int j = 0; j < buf[i].otop; j++) {
+                const Overlap* o = buf[i].ovls + j;
+                if (!only_identity || o->aread == o->bread)
                     keys.push_back(Key{o->aread, o->bread, (int)COMP(o->flags), o->path.abpos, o});
             }
     }
@@ -341,10 +437,9 @@ void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblo
     all.resize(keys.size());
     for (size_t i = 0; i < keys.size(); i++) all[i] = keys[i].o;
     std::string aroot, broot;
-    const int aid = split_block_name(ablock, aroot);
-    const int bid = split_block_name(bblock, broot);
-    const int tspace = s->hs.tspace;
-    const int tbytes = s->iobuf[0].tbytes;
+    const int aid = split_block_name(ablock.c_str(), aroot);
+    const int bid = split_block_name(bblock.c_str(), broot);
+    const int tbytes = buf[0].tbytes;
     auto name = [&](const std::string& r1, int i1, const std::string& r2, int i2) {
         std::string p;
         if (i1 > 0) p = run_dir(RUN_ID, i1) + "/" + r1 + "." + std::to_string(i1) + "." + r2;
@@ -352,29 +447,71 @@ void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblo
         if (i2 > 0) p += "." + std::to_string(i2);
         return p + ".las";
     };
-    bool ok;
-    if (strcmp(ablock, bblock) == 0 || s->symmetric == 0) {
-        ok = write_las(name(aroot, aid, broot, bid), all, 0, all.size(), tspace, tbytes);
-    } else {
-        std::string p1 = name(aroot, aid, broot, bid), p2 = name(broot, bid, aroot, aid);
-        if (bid < aid) std::swap(p1, p2);
-        size_t j = 0;
-        while (j < all.size() && all[j]->aread <= lastRead) j++;
-        ok = write_las(p1, all, 0, j, tspace, tbytes) && write_las(p2, all, j, all.size(), tspace, tbytes);
-    }
-    if (!ok) {
-        fprintf(stderr, "[ERROR] - Write_Overlap_Buffer: write failed\n");
-        exit(1);
-    }
+    if (ablock == bblock || symmetric == 0)
+        return write_las(name(aroot, aid, broot, bid), all, 0, all.size(), tspace, tbytes);
+    std::string p1 = name(aroot, aid, broot, bid), p2 = name(broot, bid, aroot, aid);
+    if (bid < aid) std::swap(p1, p2);
+    size_t j = 0;
+    while (j < all.size() && all[j]->aread <= lastRead) j++;
+    return write_las(p1, all, 0, j, tspace, tbytes) && write_las(p2, all, j, all.size(), tspace, tbytes);
 }
 
+}  // namespace
+
+extern "C" {
+
+// The records go to the background writer (a snapshot of the buffer
+// headers; the buffers themselves stay untouched until it is done): the
+// call returns once the previous write has finished.
+void Write_Overlap_Buffer(Align_Spec* spec, int RUN_ID, char* ablock, char* bblock, int lastRead)
+{
+    SpecObj* s = (SpecObj*)spec;
+    mi::wait_overlap_writes();
+    auto snap = std::make_shared<std::vector<Overlap_IO_Buffer>>(s->iobuf, s->iobuf + s->nthreads);
+    std::vector<const Overlap*> arrays;
+    for (const auto& b : *snap) arrays.push_back(b.ovls);
+    const int nt = s->nthreads, oi = s->only_identity, sym = s->symmetric, ts = s->hs.tspace;
+    const std::string a(ablock), b(bblock);
+    writer().submit([=] { return write_pair(snap->data(), nt, oi, sym, ts, RUN_ID, a, b, lastRead); },
+                    std::move(arrays));
+    s->written = true;
+}
+
+// Empties the buffers.  After a write the writer may still be reading them:
+// they trade places with the spare set (made here the first time), which
+// the previous write finished with.
 void Reset_Overlap_Buffer(Align_Spec* spec)
 {
     SpecObj* s = (SpecObj*)spec;
+    if (s->written) {
+        if (s->spare == nullptr) {
+            s->spare = (Overlap_IO_Buffer*)malloc(sizeof(Overlap_IO_Buffer) * s->nthreads);
+            for (int i = 0; s->spare && i < s->nthreads; i++) {
+                Overlap_IO_Buffer* ob = CreateOverlapBuffer(s->nthreads, s->iobuf[i].tbytes ? s->iobuf[i].tbytes : 1,
+                                                            s->iobuf[i].no_trace);
+                if (ob == nullptr) {
+                    fprintf(stderr, "[ERROR] - Reset_Overlap_Buffer: out of memory\n");
+                    exit(1);
+                }
+                s->spare[i] = *ob;
+                free(ob);
+            }
+            if (s->spare == nullptr) {
+                fprintf(stderr, "[ERROR] - Reset_Overlap_Buffer: out of memory\n");
+                exit(1);
+            }
+        }
+        for (int i = 0; i < s->nthreads; i++) std::swap(s->iobuf[i], s->spare[i]);
+        s->written = false;
+    }
     for (int i = 0; i < s->nthreads; i++) {
         s->iobuf[i].otop = 0;
         s->iobuf[i].ttop = 0;
     }
 }
+
+// Waits for the .las files handed to the background writer; 0, or 1 when
+// one of them failed to write.
+int mi_flush_overlap_writes(void) { return writer().wait() ? 0 : 1; }
 
 }  // extern "C"

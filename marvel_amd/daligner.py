@@ -90,6 +90,7 @@ class Daligner:
             "mi_release_tracks": ([db], None),
             "Write_Overlap_Buffer": ([vp, i, C.c_char_p, C.c_char_p, i], None),
             "Reset_Overlap_Buffer": ([vp], None),
+            "mi_flush_overlap_writes": ([], i),
             "mi_last_error": ([], C.c_char_p),
         }
         for name, (args, res) in sig.items():
@@ -194,8 +195,16 @@ class Daligner:
     def free_index(self, t):
         self.h.mi_free_kmer_index(t)
 
-    def write_overlap_buffer(self, spec, run_id, aname: bytes, bname: bytes, last_read: int):
+    def write_overlap_buffer(self, spec, run_id, aname: bytes, bname: bytes, last_read: int, wait: bool = True):
+        """Write_Overlap_Buffer; the write runs in the background, so with
+        wait=False the .las files are complete only after flush_writes()."""
         self.h.Write_Overlap_Buffer(spec, run_id, aname, bname, last_read)
+        if wait:
+            self.flush_writes()
+
+    def flush_writes(self):
+        if self.h.mi_flush_overlap_writes():
+            raise DalignerError("a .las write failed")
 
     def reset_overlap_buffer(self, spec):
         self.h.Reset_Overlap_Buffer(spec)
@@ -204,11 +213,13 @@ class Daligner:
         if self.h.mi_device_synchronize():
             self._err("device synchronize")
 
-    def compare_self(self, A: HITS_DB, spec, run_id: int, name: str, dev=None, cA=None):
+    def compare_self(self, A: HITS_DB, spec, run_id: int, name: str, dev=None, cA=None, wait: bool = True):
         """One block against itself, both strands, .las written in the
         current directory's run folder (dalign/daligner.c:994-1024).  With
         `dev` (a mi_block_device handle) the index is built from reads
-        already in HBM.  Returns (forward stats, complement stats)."""
+        already in HBM.  wait=False leaves the .las write running in the
+        background (overlapping the next call; flush_writes() waits for it).
+        Returns (forward stats, complement stats)."""
         nm = C.create_string_buffer(name.encode())
         os.makedirs("%03d_%05d" % (run_id, A.part), exist_ok=True)   # createSubdir (dalign/daligner.c:583-611)
         t = [time.perf_counter()]
@@ -229,7 +240,7 @@ class Daligner:
                     self.release_tracks(cA)   # the complemented mask, if A has one
         finally:
             self.free_index(a)
-        self.write_overlap_buffer(spec, run_id, nm, nm, A.ufirst + A.nreads - 1)
+        self.write_overlap_buffer(spec, run_id, nm, nm, A.ufirst + A.nreads - 1, wait=wait)
         self.reset_overlap_buffer(spec)
         t.append(time.perf_counter())
         self.last_phases = {k: round(t[i + 1] - t[i], 4) for i, k in

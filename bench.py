@@ -160,8 +160,8 @@ def cpu_baseline(reads, d):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gated", action="store_true", help="skip the scorer-gated leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -50,8 +50,10 @@ CASES = {
     "rep_mask_blocks": ("small_rep", 1, ["-mdust", "-mrnd", "-mpt", "-j8", "-M16"], ["C1.2", "C1.1", "C1.3"]),
     # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)
     "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
+    # the same with the reference DBdust track of its reads as a soft mask
+    "c2_mdust": ("5mbp", 400, ["-mdust", "-j8", "-M32"], ["C2.1", "C2.1"]),
 }
-LARGE = {"c2_j8"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
+LARGE = {"c2_j8", "c2_mdust"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
 
 # -m tracks a case's DB carries: name -> (layout, source).  "dust" is the
 # reference DBdust output for the reads (a fixture); "rnd<seed>" random
@@ -62,6 +64,7 @@ MASKS = {
     "rep_mdust_mrnd": {"dust": ("dazzler", "dust"), "rnd": ("dazzler", "rnd5")},
     "rep_mm2": {"m2": ("marvel", "rnd6")},
     "rep_mask_blocks": {"dust": ("dazzler", "dust"), "rnd": ("dazzler", "rnd7"), "pt": ("dazzler", "part8")},
+    "c2_mdust": {"dust": ("dazzler", "dust")},
 }
 MASK_FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mask")
 

@@ -83,8 +83,10 @@ int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin);
  * *len = number of k-mers after -t suppression.  With block->tracks set,
  * only k-mers wholly outside the first track's intervals are indexed
  * (tuple_thread's masked branch, dalign/filter.c:416-470); a track whose
- * points are not increasing pairs within the reads is an error.  BIASED
- * (-b) is not supported.  The handle is released by Match_Filter when passed as its
+ * points are not increasing pairs within the reads is an error.  With
+ * BIASED (-b) the k-mers are biased_tuple_thread's (dalign/filter.c:492-633),
+ * weighted by the base frequencies of the first block indexed in the
+ * process, as the reference computes them once.  The handle is released by Match_Filter when passed as its
  * btable with atable != btable (the reference frees its B index there,
  * dalign/filter.c:2194-2202), otherwise by mi_free_kmer_index. */
 void *Sort_Kmers(HITS_DB *block, int *len);

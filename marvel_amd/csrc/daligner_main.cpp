@@ -2,7 +2,7 @@
 // dalign/daligner.c:615-1032 over libmarvl_align's Sort_Kmers /
 // Match_Filter / Write_Overlap_Buffer.
 //
-//   daligner [-vAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]
+//   daligner [-vbAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]
 //            [-e<double(.70)>] [-l<int(1000)>] [-s<int(100)>] [-H<int>]
 //            [-r<int(1)>] [-j<int(4)>] [-m<track>]+ <subject:db> <target:db> ...
 //
@@ -10,8 +10,8 @@
 // 2), which decides the pairs the reference skips at chunk ends; the GPU
 // work itself is not split by it.  -m<track> (repeatable) soft-masks the
 // intervals of the named tracks, Dazzler (.anno/.data) or MARVEL
-// (.a2/.d2) layout.  -b (biased k-mers), -I / -O (identity) and .dam inputs
-// are not supported.
+// (.a2/.d2) layout; -b weighs k-mers by the base frequencies.  -I / -O
+// (identity) and .dam inputs are not supported.
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -50,7 +50,7 @@ static void make_subdir(int run_id, int part)
 static void usage()
 {
     fprintf(stderr,
-            "usage: daligner [-vAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]\n"
+            "usage: daligner [-vbAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]\n"
             "                [-e<double(.70)>] [-l<int(1000)>] [-s<int(100)>] [-H<int>] [-r<int(1)>] [-j<int(4)>]\n"
             "                [-m<track>]+\n"
             "                <subject:db> <target:db> ...\n");
@@ -85,6 +85,7 @@ int main(int argc, char** argv)
     while ((c = getopt(argc, argv, "vbOTAIk:w:h:t:M:e:l:s:H:D:m:r:j:")) != -1) {
         switch (c) {
         case 'v': VERBOSE += 1; break;
+        case 'b': BIASED = 1; break;
         case 'T': no_trace = 1; break;
         case 'A': SYMMETRIC = 0; break;
         case 'k': kmer = atoi(optarg); break;

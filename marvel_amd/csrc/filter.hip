@@ -258,6 +258,7 @@ constexpr uint32_t kIdxMagic = 0x4d49584bu;   // "KXIM"
 struct KIdx {
     uint32_t magic = kIdxMagic;
     int nreads = 0, maxlen = 0, ufirst = 0, part = 0, K = 0;
+    float freq[4] = {.25f, .25f, .25f, .25f};   // the block's base frequencies (-b weights)
     KeyFmt fmt{1, 1};
     long long nk = 0;
     long long nwords = 0;
@@ -282,6 +283,7 @@ static void layout_reads(KIdx* x, const HITS_DB* blk)
     x->maxlen = blk->maxlen;
     x->ufirst = blk->ufirst;
     x->part = blk->part;
+    for (int i = 0; i < 4; i++) x->freq[i] = blk->freq[i];
     x->rlen.resize(x->nreads);
     x->w0.resize(x->nreads);
     long long wpos = 1;
@@ -330,6 +332,86 @@ static void take_mask(KIdx* x, const HITS_DB* blk)
     }
 }
 
+static void suppress_frequent(KIdx* x, DBuf<uint64_t>& k2);
+
+// -b weights: ceil(-10000 log4 freq) per base, fixed by the first biased
+// index of the process (dalign/filter.c:718-732 computes them once, from the
+// block Sort_Kmers first sees, and keeps them for every later block)
+static bool g_bias_set = false;
+static int g_logbase[4];
+
+// the -b k-mer list of x into x->keys (unsorted, in read / position order); returns the count
+static long long biased_keys(KIdx* x, int K)
+{
+    hipStream_t st = fstream();
+    if (!g_bias_set) {
+        const double scale = -10000. / log(4.);
+        for (int i = 0; i < 4; i++) {
+            if (!(x->freq[i] > 0.f && x->freq[i] < 1.f))
+                throw Fail("-b needs every base frequency of the block in (0, 1)", MI_ERR_ARG);
+            g_logbase[i] = (int)ceil(scale * log(x->freq[i]));
+        }
+        g_bias_set = true;
+    }
+    // runs: whole reads, or with a mask the gaps that hold a K-mer
+    const bool masked = !x->manno.empty();
+    std::vector<int> rr, rp, rq;
+    for (int i = 0; i < x->nreads; i++) {
+        if (!masked) {
+            rr.push_back(i);
+            rp.push_back(0);
+            rq.push_back(x->rlen[i]);
+            continue;
+        }
+        const long long b = x->manno[i], f = x->manno[i + 1];
+        for (long long a = b; a <= f; a += 2) {
+            const int p = (a == b) ? 0 : x->mpts[a - 1];
+            const int q = (a == f) ? x->rlen[i] : x->mpts[a];
+            if (p + K > q) continue;
+            rr.push_back(i);
+            rp.push_back(p);
+            rq.push_back(q);
+        }
+    }
+    const int nrun = (int)rr.size();
+    if (nrun == 0) return 0;
+    BiasParams bp;
+    for (int i = 0; i < 4; i++) bp.lb[i] = g_logbase[i];
+    bp.lognorm = 10000 * K;
+    bp.logthresh = 10000 * (K - 2);   // MAX_BIAS 2 (dalign/filter.c:18)
+    bp.masked = masked ? 1 : 0;
+    DBuf<int> drr, drp, drq, dcnt;
+    DBuf<long long> dout;
+    drr.ensure(nrun);
+    drp.ensure(nrun);
+    drq.ensure(nrun);
+    dcnt.ensure(nrun);
+    dout.ensure(nrun);
+    HC(hipMemcpyAsync(drr.p, rr.data(), sizeof(int) * nrun, hipMemcpyHostToDevice, st));
+    HC(hipMemcpyAsync(drp.p, rp.data(), sizeof(int) * nrun, hipMemcpyHostToDevice, st));
+    HC(hipMemcpyAsync(drq.p, rq.data(), sizeof(int) * nrun, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_biased<false>), dim3(nblk(nrun, 64)), dim3(64), 0, st, x->pk.p, x->dw0.p, drr.p,
+                       drp.p, drq.p, nrun, K, bp, x->fmt, nullptr, dcnt.p, nullptr);
+    HC(hipGetLastError());
+    std::vector<int> cnt(nrun);
+    HC(hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int) * nrun, hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    std::vector<long long> ofs(nrun);
+    long long nk = 0;
+    for (int s = 0; s < nrun; s++) {
+        ofs[s] = nk;
+        nk += cnt[s];
+    }
+    x->keys.ensure(nk + 1);
+    if (nk > 0) {
+        HC(hipMemcpyAsync(dout.p, ofs.data(), sizeof(long long) * nrun, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_biased<true>), dim3(nblk(nrun, 64)), dim3(64), 0, st, x->pk.p, x->dw0.p,
+                           drr.p, drp.p, drq.p, nrun, K, bp, x->fmt, dout.p, nullptr, x->keys.p);
+        HC(hipGetLastError());
+    }
+    return nk;
+}
+
 static void build_keys(KIdx* x)
 {
     hipStream_t st = fstream();
@@ -337,6 +419,18 @@ static void build_keys(KIdx* x)
     x->K = K;
     if (2 * K + x->fmt.rbits + x->fmt.pbits > 64)
         throw Fail("block too large for 64-bit k-mer keys (2k + read bits + position bits > 64)", MI_ERR_ARG);
+    if (BIASED) {
+        const long long nk = biased_keys(x, K);
+        DBuf<uint64_t> k2;
+        k2.ensure(nk + 1);
+        const int cs = x->fmt.cshift();
+        uint64_t* sorted = radix_sort(x->keys.p, k2.p, nk, cs, cs + 2 * K, KeyU64{});
+        if (sorted != x->keys.p) x->keys.swap(k2);
+        x->nk = nk;
+        suppress_frequent(x, k2);
+        HC(hipStreamSynchronize(st));
+        return;
+    }
     // runs of consecutive k-mers: one per read, or with a mask one per gap
     // between masked intervals that holds a k-mer, [0, pt0), [pt1, pt2), ...,
     // [ptlast, rlen) (dalign/filter.c:416-452); kofs[s] = k-mers before run s
@@ -389,7 +483,17 @@ static void build_keys(KIdx* x)
     uint64_t* sorted = radix_sort(x->keys.p, k2.p, nk, cs, cs + 2 * K, KeyU64{});
     if (sorted != x->keys.p) x->keys.swap(k2);
     x->nk = nk;
-    if (g_suppress > 0 && nk > 0) {   // -t: drop groups of >= t entries (dalign/filter.c:645-696)
+    suppress_frequent(x, k2);
+    HC(hipStreamSynchronize(st));
+}
+
+// -t: drop the k-mer groups of >= t entries from the sorted keys (dalign/filter.c:645-696);
+// k2 is scratch of at least nk + 1 keys
+static void suppress_frequent(KIdx* x, DBuf<uint64_t>& k2)
+{
+    hipStream_t st = fstream();
+    const long long nk = x->nk;
+    if (g_suppress > 0 && nk > 0) {
         DBuf<u64> flag, scan;
         DBuf<long long> gidx, gstart;
         flag.ensure(nk);
@@ -462,6 +566,7 @@ static KIdx* index_from_device(const KIdx* src)
     x->maxlen = src->maxlen;
     x->ufirst = src->ufirst;
     x->part = src->part;
+    for (int i = 0; i < 4; i++) x->freq[i] = src->freq[i];
     x->fmt = src->fmt;
     x->rlen = src->rlen;
     x->w0 = src->w0;
@@ -1285,7 +1390,6 @@ int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin)
 static void* sort_kmers_impl(HITS_DB* block, int* len, const KIdx* comp_of)
 {
     try {
-        if (BIASED) throw Fail("biased k-mer selection (-b) is not supported", MI_ERR_ARG);
         KIdx* x = comp_of ? index_complement(comp_of, block) : index_from_block(block);
         if (len) *len = (int)std::min<long long>(x->nk, INT32_MAX);
         return x;

@@ -117,6 +117,70 @@ __global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __rest
     keys[i] = (code << fmt.cshift()) | ((uint64_t)r << fmt.pbits) | (uint64_t)rpos;
 }
 
+// -b k-mers (biased_tuple_thread, dalign/filter.c:492-633), one lane per
+// run of positions [p, q) of a read: a window grows forward until its weight
+// (the sum of its bases' ceil(-10000 log4 freq)) reaches 10000 K, sheds its
+// oldest bases while it stays there, and yields its code left-aligned to 2K
+// bits, at the window's last position, when its weight exceeds 10000 (K-2).
+// Unmasked runs are whole reads, and a window that runs into the read's end
+// ends it; masked runs stop at q and may still yield a k-mer at position q.
+// Pass 1 (EMIT false) counts each run's k-mers, pass 2 writes their keys
+// from out[run] on.
+struct BiasParams {
+    int lb[4];
+    int lognorm, logthresh, masked;
+};
+
+template <bool EMIT>
+__global__ void k_biased(const uint64_t* __restrict__ pk, const long long* __restrict__ w0, const int* __restrict__ rread,
+                         const int* __restrict__ rp, const int* __restrict__ rq, int nrun, int K, BiasParams bp,
+                         KeyFmt fmt, const long long* __restrict__ out, int* __restrict__ counts,
+                         uint64_t* __restrict__ keys)
+{
+    const uint64_t kmask = (1ull << (2 * K)) - 1;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nrun; s += gridDim.x * blockDim.x) {
+        const int r = rread[s], q = rq[s];
+        int p = rp[s];
+        const uint64_t* w = pk + w0[r];
+        auto base = [&](int i) { return (int)((w[i >> 5] >> (2 * (i & 31))) & 3); };
+        long long o = EMIT ? out[s] : 0;
+        int n = 0, a = 0, k = 1;
+        uint64_t c = 0;
+        bool end = false;
+        while (p < q) {
+            int x = base(p);
+            a += bp.lb[x];
+            c = (c << 2) | (uint64_t)x;
+            while (a < bp.lognorm && k < K) {
+                if (++p >= q) {
+                    end = !bp.masked;
+                    break;
+                }
+                k += 1;
+                x = base(p);
+                a += bp.lb[x];
+                c = (c << 2) | (uint64_t)x;
+            }
+            if (end) break;
+            while (k > 1) {   // (positive weights never shed the last base)
+                const int u = a - bp.lb[base(p - k + 1)];
+                if (u < bp.lognorm) break;
+                a = u;
+                k -= 1;
+            }
+            if (a > bp.logthresh) {
+                if (EMIT)
+                    keys[o + n] = (((c << (2 * K - 2 * k)) & kmask) << fmt.cshift()) | ((uint64_t)r << fmt.pbits) |
+                                  (uint64_t)p;
+                n++;
+            }
+            p += 1;
+            a -= bp.lb[base(p - k)];
+        }
+        if (!EMIT) counts[s] = n;
+    }
+}
+
 // group starts of a sorted key list (by code)
 __global__ void k_group_flags(const uint64_t* __restrict__ keys, long long n, KeyFmt fmt,
                               unsigned long long* __restrict__ flag)

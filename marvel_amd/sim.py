@@ -143,6 +143,15 @@ def config_small_repeats(seed: int = 11):
     return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
 
 
+def config_at_rich(seed: int = 31):
+    """config_small's read model over a 200 kbp genome of skewed composition
+    (A, C, G, T at 35 / 15 / 15 / 35 %): the base frequencies daligner -b
+    weighs k-mers by."""
+    rng = np.random.default_rng(seed)
+    g = rng.choice(4, size=200_000, p=[0.35, 0.15, 0.15, 0.35]).astype(np.uint8)
+    return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
+
+
 def config_band_edges(seed: int = 21):
     """Reads that drive report_thread's outermost diagonal bands: two reads of
     the block's maximum length X1, X2 (4,990 bp); Y begins with X1's last 60 bp

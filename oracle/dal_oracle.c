@@ -9,6 +9,7 @@
 
 #include <errno.h>
 #include <limits.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1267,17 +1268,86 @@ static void radix_kmers(kmer_t **src, kmer_t **tmp, int64_t n, int bits)
     free(cnt);
 }
 
-static kmer_t *sort_kmers(const or_block *b, int K, int suppress, int64_t *nk_out)
+/* -b weights (dalign/filter.c:718-732): per base ceil(-10000 log4 freq),
+ * computed once, from the block of the first biased index a process
+ * builds, and kept for every later block (its complement included). */
+static int bias_set, LogBase[4];
+
+/* biased_tuple_thread for one run of positions [p, q) of a read
+ * (dalign/filter.c:492-633): a window grows forward until its weight reaches
+ * 10000 K, sheds its oldest bases while it stays there, and yields the
+ * window's code left-aligned to 2K bits when its weight exceeds
+ * 10000 (K - 2).  Without a mask (masked == 0) a window that runs into the
+ * read's end ends the read; with one it stops at q and may still yield a
+ * k-mer ending at q. */
+static int64_t biased_run(const char *s, int p, int q, int K, int masked, int read, kmer_t *list, int64_t m)
+{
+    const uint64_t kmask = (K >= 32) ? ~0ULL : ((1ULL << (2 * K)) - 1);
+    const int lognorm = 10000 * K, logthresh = 10000 * (K - 2);
+    const char *t = s + 1;
+    uint64_t c = 0;
+    int a = 0, k = 1, x;
+    while (p < q) {
+        x = s[p];
+        a += LogBase[x];
+        c = (c << 2) | (uint64_t)x;
+        while (a < lognorm && k < K) {
+            if (++p >= q) {
+                if (!masked) return m;
+                break;
+            }
+            k += 1;
+            x = s[p];
+            a += LogBase[x];
+            c = (c << 2) | (uint64_t)x;
+        }
+        for (;;) {
+            int u = a - LogBase[(int)t[p - k]];
+            if (u < lognorm) break;
+            a = u;
+            k -= 1;
+        }
+        if (a > logthresh) {
+            list[m].code = (c << (2 * K - 2 * k)) & kmask;
+            list[m].read = read;
+            list[m].rpos = p;
+            m++;
+        }
+        p += 1;
+        a -= LogBase[(int)s[p - k]];
+    }
+    return m;
+}
+
+static kmer_t *sort_kmers(const or_block *b, int K, int suppress, int biased, int64_t *nk_out)
 {
     int64_t n = b->boff[b->nreads] - (int64_t)K * b->nreads, m = 0;
     uint64_t kmask = (K >= 32) ? ~0ULL : ((1ULL << (2 * K)) - 1);
     if (n <= 0) { *nk_out = 0; return NULL; }
     kmer_t *list = (kmer_t *)xrealloc(NULL, sizeof(kmer_t) * (n + 1));
     kmer_t *tmp = (kmer_t *)xrealloc(NULL, sizeof(kmer_t) * (n + 1));
+    if (biased && !bias_set) {
+        const double scale = -10000. / log(4.);
+        for (int i = 0; i < 4; i++) LogBase[i] = (int)ceil(scale * log(b->freq[i]));
+        bias_set = 1;
+    }
     for (int r = 0; r < b->nreads; r++) {
         const char *s = b->bases + b->boff[r];
         uint64_t c = 0;
         int p = 0, x;
+        if (biased) {
+            if (b->manno == NULL) {
+                m = biased_run(s, 0, b->rlen[r], K, 0, r, list, m);
+                continue;
+            }
+            for (int64_t a = b->manno[r]; a <= b->manno[r + 1]; a += 2) {
+                int q;
+                p = (a == b->manno[r]) ? 0 : b->mdata[a - 1];
+                q = (a == b->manno[r + 1]) ? b->rlen[r] : b->mdata[a];
+                if (p + K <= q) m = biased_run(s, p, q, K, 1, r, list, m);
+            }
+            continue;
+        }
         if (b->manno) {
             /* masked: only k-mers wholly inside a gap between masked
              * intervals, [0, pt0), [pt1, pt2), ..., [ptlast, rlen)
@@ -1906,25 +1976,25 @@ int or_daligner_pair(const or_opts *o, const or_spec *sp, or_block *A, or_block 
     int64_t na, nb;
     or_work *w = or_work_new();
     out->tbytes = (o->tspace <= 125) ? 1 : 2;
-    kmer_t *as = sort_kmers(A, o->kmer, o->suppress, &na);
+    kmer_t *as = sort_kmers(A, o->kmer, o->suppress, o->biased, &na);
     st->nkmers_a = na;
     if (same_file) {
         or_daligner_pass(o, sp, A, A, 1, 0, as, na, as, na, 1, out, st, w);
         or_block cB;
         or_block_complement(A, &cB);
-        kmer_t *bs = sort_kmers(&cB, o->kmer, o->suppress, &nb);
+        kmer_t *bs = sort_kmers(&cB, o->kmer, o->suppress, o->biased, &nb);
         st->nkmers_b = nb;
         or_daligner_pass(o, sp, A, &cB, 1, 1, as, na, bs, nb, 0, out, st, w);
         free(bs);
         or_block_free(&cB);
     } else {
-        kmer_t *bs = sort_kmers(B, o->kmer, o->suppress, &nb);
+        kmer_t *bs = sort_kmers(B, o->kmer, o->suppress, o->biased, &nb);
         st->nkmers_b = nb;
         or_daligner_pass(o, sp, A, B, 0, 0, as, na, bs, nb, 0, out, st, w);
         free(bs);
         or_block cB;
         or_block_complement(B, &cB);
-        bs = sort_kmers(&cB, o->kmer, o->suppress, &nb);
+        bs = sort_kmers(&cB, o->kmer, o->suppress, o->biased, &nb);
         or_daligner_pass(o, sp, A, &cB, 0, 1, as, na, bs, nb, 0, out, st, w);
         free(bs);
         or_block_free(&cB);

@@ -88,6 +88,7 @@ void or_block_complement(const or_block *src, or_block *dst);
 typedef struct {
     int kmer, binshift, hitmin, suppress, minover, tspace, nthreads, nshift;
     int hgap_min, symmetric, run_id, no_trace, verbose;
+    int biased;   /* -b: k-mers of variable length weighted by the base frequencies */
     double ave_corr;
     uint64_t mem_limit;
 } or_opts;

@@ -61,6 +61,7 @@ int main(int argc, char **argv)
     while ((c = getopt(argc, argv, "vbOTAIk:w:h:t:M:e:l:s:H:D:m:r:j:")) != -1) {
         switch (c) {
         case 'v': o.verbose += 1; break;
+        case 'b': o.biased = 1; break;
         case 'T': o.no_trace = 1; break;
         case 'A': o.symmetric = 0; break;
         case 'k': o.kmer = atoi(optarg); break;

@@ -20,6 +20,7 @@ DATA = {
     "small_rep": lambda: sim.config_small_repeats(11),
     "5mbp": lambda: sim.config_5mbp(2),
     "edges": lambda: sim.config_band_edges(21),
+    "at_rich": lambda: sim.config_at_rich(31),
 }
 
 # name -> (data, DBsplit size in Mbp, daligner flags, block arguments)
@@ -48,6 +49,14 @@ CASES = {
     "rep_mdust_mrnd": ("small_rep", 400, ["-mdust", "-mrnd", "-j4", "-M16"], ["C1.1", "C1.1"]),
     "rep_mm2": ("small_rep", 400, ["-mm2", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "rep_mask_blocks": ("small_rep", 1, ["-mdust", "-mrnd", "-mpt", "-j8", "-M16"], ["C1.2", "C1.1", "C1.3"]),
+    # biased k-mers (-b): variable-length k-mers of high specificity under skewed base frequencies (on
+    # uniform ones they are the plain 14-mers); unbiased for comparison, with a mask (k-mers can end one past
+    # a gap), and over blocks (the weights come from the first block indexed, A)
+    "c1_b": ("small", 400, ["-b", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "at_j8": ("at_rich", 400, ["-j8", "-M16"], ["C1.1", "C1.1"]),
+    "at_b": ("at_rich", 400, ["-b", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "at_b_mrnd": ("at_rich", 400, ["-b", "-mrnd", "-j4", "-M16"], ["C1.1", "C1.1"]),
+    "at_b_blocks": ("at_rich", 1, ["-b", "-j8", "-M16"], ["C1.3", "C1.1", "C1.2"]),
     # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)
     "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
     # the same with the reference DBdust track of its reads as a soft mask
@@ -65,6 +74,7 @@ MASKS = {
     "rep_mm2": {"m2": ("marvel", "rnd6")},
     "rep_mask_blocks": {"dust": ("dazzler", "dust"), "rnd": ("dazzler", "rnd7"), "pt": ("dazzler", "part8")},
     "c2_mdust": {"dust": ("dazzler", "dust")},
+    "at_b_mrnd": {"rnd": ("dazzler", "rnd9")},
 }
 MASK_FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mask")
 

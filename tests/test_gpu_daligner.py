@@ -54,7 +54,7 @@ def test_gpu_daligner_matches_reference(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["c1_j8", "rep_j8", "rep_mdust_mrnd", "c2_j8"])
+@pytest.mark.parametrize("case", ["c1_j8", "rep_j8", "rep_mdust_mrnd", "at_b", "c2_j8"])
 def test_gpu_daligner_in_process(case, tmp_path):
     """Same comparison through the library loaded in this process (Python
     mirror of Set_Filter_Params / Sort_Kmers / Match_Filter /
@@ -68,7 +68,8 @@ def test_gpu_daligner_in_process(case, tmp_path):
     las_cases.make_db(case, str(tmp_path))
     dal = Daligner()
     nthreads = int(opts.get("j", "4"))
-    dal.configure(nthreads=nthreads, mem_limit_gb=int(opts["M"]), minover=2 * int(opts.get("l", "1000")))
+    dal.configure(nthreads=nthreads, mem_limit_gb=int(opts["M"]), minover=2 * int(opts.get("l", "1000")),
+                  biased=int("b" in opts))
     dal.set_filter_params(int(opts.get("k", "14")), int(opts.get("w", "6")), int(opts.get("t", "0")),
                           int(opts.get("h", "35")))
     A = dal.open_block(str(tmp_path / blocks[0]))
@@ -91,7 +92,7 @@ INTEG = os.path.join(ROOT, "oracle", "_ref", "daligner_on_libmarvl")
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(INTEG), reason="integration binary not built (make -C oracle integ)")
-@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "rep_mdust_mrnd", "rep_mask_blocks", "c2_j8"])
+@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "rep_mdust_mrnd", "rep_mask_blocks", "at_b_mrnd", "c2_j8"])
 def test_reference_driver_on_library(case):
     """The reference's own daligner.c and DB code, linked against
     libmarvl_align instead of its filter.c / align.c, writes the golden .las.

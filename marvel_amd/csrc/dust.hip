@@ -1,8 +1,9 @@
 // DBdust on the GPU: the SDUST low-complexity scan of db/DBdust.c:236-470
 // (Morgulis et al., JCB 13(5) 2006) over every read of a block, one lane per
 // read.  The scan is sequential along a read, so the parallelism is across
-// reads; each lane keeps its triplet counts in LDS ([triplet][lane], a bank
-// per lane) and its candidate list and interval stack in global memory.
+// reads; each lane keeps its triplet counts and (for windows up to 64) its
+// candidate list in LDS ([item][lane], a bank per lane), its interval stack
+// in global memory.
 //
 // Per read (triplet codes t[0 .. len-3] of bases j, j+1, j+2):
 //   * the window counts cover (wb, j], the scanned run (lb, j]; wsqr / lsqr
@@ -53,9 +54,19 @@ struct DustArgs {
     int* status;                 // out: nonzero when a read ran out of candidate nodes
 };
 
+// Candidate nodes live in LDS when the window is at most kDustLdsWindow
+// (the default 64): every step of the scan touches them, and HBM latency
+// would serialise each lane's walk.  Larger windows keep them in HBM.
+constexpr int kDustLdsWindow = 64;
+
+template <bool LDS_NODES>
 __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
 {
     __shared__ int wcnt[64 * 64], lcnt[64 * 64];   // [triplet][lane]
+    constexpr int kN = LDS_NODES ? (kDustLdsWindow + 1) * 64 : 1;
+    __shared__ int sbeg[kN], send[kN];
+    __shared__ double ssc[kN];
+    __shared__ short snx[kN], spv[kN];
     const int lane = threadIdx.x;
     const long long slot = (long long)blockIdx.x * 64 + lane;
     const long long S = a.nslots;
@@ -65,11 +76,11 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
         const int L = a.rlen[r] - 2;
         auto T = [&](int i) { return (int)((b[i] << 4) | (b[i + 1] << 2) | b[i + 2]); };
         // candidate nodes: 0 is the list head (beg = -2), 1..window the free list
-        auto BEG = [&](int n) -> int& { return a.cbeg[n * S + slot]; };
-        auto END = [&](int n) -> int& { return a.cend[n * S + slot]; };
-        auto SC = [&](int n) -> double& { return a.cscore[n * S + slot]; };
-        auto NX = [&](int n) -> short& { return a.cnext[n * S + slot]; };
-        auto PV = [&](int n) -> short& { return a.cprev[n * S + slot]; };
+        auto BEG = [&](int n) -> int& { return LDS_NODES ? sbeg[n * 64 + lane] : a.cbeg[n * S + slot]; };
+        auto END = [&](int n) -> int& { return LDS_NODES ? send[n * 64 + lane] : a.cend[n * S + slot]; };
+        auto SC = [&](int n) -> double& { return LDS_NODES ? ssc[n * 64 + lane] : a.cscore[n * S + slot]; };
+        auto NX = [&](int n) -> short& { return LDS_NODES ? snx[n * 64 + lane] : a.cnext[n * S + slot]; };
+        auto PV = [&](int n) -> short& { return LDS_NODES ? spv[n * 64 + lane] : a.cprev[n * S + slot]; };
         NX(0) = 0;
         PV(0) = 0;
         BEG(0) = -2;
@@ -301,7 +312,8 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
     const int cus = std::max(1, mi_num_cus());
     const int grid = (int)std::min<long long>((n + 63) / 64, (long long)cus * 8);
     const long long nslots = (long long)grid * 64;
-    const long long nodes = (long long)(window + 1) * nslots;
+    const bool lds_nodes = window <= kDustLdsWindow;
+    const long long nodes = lds_nodes ? 1 : (long long)(window + 1) * nslots;
     void *dbases = nullptr, *dboff = nullptr, *drlen = nullptr, *dskew = nullptr, *dbeg = nullptr, *dend = nullptr,
          *dsc = nullptr, *dnx = nullptr, *dpv = nullptr, *dstk = nullptr, *dsoff = nullptr, *dnp = nullptr,
          *dst = nullptr;
@@ -344,7 +356,8 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
             hipMemcpy(dskew, skew, sizeof skew, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(dsoff, soff.data(), 8 * (n + 1), hipMemcpyHostToDevice) == hipSuccess;
         if (ok) {
-            hipLaunchKernelGGL(k_sdust, dim3(grid), dim3(64), 0, 0, a);
+            if (lds_nodes) hipLaunchKernelGGL(k_sdust<true>, dim3(grid), dim3(64), 0, 0, a);
+            else hipLaunchKernelGGL(k_sdust<false>, dim3(grid), dim3(64), 0, 0, a);
         }
         stk.resize(tot);
         if (!ok || hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -345,11 +346,11 @@ static long long biased_keys(KIdx* x, int K)
 {
     hipStream_t st = fstream();
     if (!g_bias_set) {
-        const double scale = -10000. / log(4.);
+        const double scale = -10000. / std::log(4.);
         for (int i = 0; i < 4; i++) {
             if (!(x->freq[i] > 0.f && x->freq[i] < 1.f))
                 throw Fail("-b needs every base frequency of the block in (0, 1)", MI_ERR_ARG);
-            g_logbase[i] = (int)ceil(scale * log(x->freq[i]));
+            g_logbase[i] = (int)std::ceil(scale * std::log((double)x->freq[i]));
         }
         g_bias_set = true;
     }

@@ -67,6 +67,12 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
     __shared__ int sbeg[kN], send[kN];
     __shared__ double ssc[kN];
     __shared__ short snx[kN], spv[kN];
+    // triplet codes of the positions the scan can still reach: a ring of
+    // kRing per lane ([slot][lane]), refilled kRing/2 positions ahead.  The
+    // scan reads only positions in (j - window, j], so a slot is rewritten
+    // only after its position left the window.
+    constexpr int kRing = LDS_NODES ? 128 : 2 * kDustMaxWindow;
+    __shared__ uint8_t ring[kRing * 64];
     const int lane = threadIdx.x;
     const long long slot = (long long)blockIdx.x * 64 + lane;
     const long long S = a.nslots;
@@ -74,7 +80,13 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
         for (int e = 0; e < 64; e++) wcnt[e * 64 + lane] = lcnt[e * 64 + lane] = 0;
         const uint8_t* b = a.bases + a.boff[r];
         const int L = a.rlen[r] - 2;
-        auto T = [&](int i) { return (int)((b[i] << 4) | (b[i + 1] << 2) | b[i + 2]); };
+        auto T = [&](int i) { return (int)ring[(i & (kRing - 1)) * 64 + lane]; };
+        auto fill = [&](int j0) {   // codes of positions [j0, j0 + kRing/2) that exist
+            const int j1 = min(j0 + kRing / 2, L);
+#pragma unroll 8
+            for (int i = j0; i < j1; i++)
+                ring[(i & (kRing - 1)) * 64 + lane] = (uint8_t)((b[i] << 4) | (b[i + 1] << 2) | b[i + 2]);
+        };
         // candidate nodes: 0 is the list head (beg = -2), 1..window the free list
         auto BEG = [&](int n) -> int& { return LDS_NODES ? sbeg[n * 64 + lane] : a.cbeg[n * S + slot]; };
         auto END = [&](int n) -> int& { return LDS_NODES ? send[n * 64 + lane] : a.cend[n * S + slot]; };
@@ -109,6 +121,7 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
         if (!a.biased) {
             int wsqr = 0, lsqr = 0;
             for (int j = 0; j < L && !bad; j++) {
+                if ((j & (kRing / 2 - 1)) == 0) fill(j);
                 int c = T(j);
                 if (j > a.window - 3) {
                     const int d = T(++wb);
@@ -177,6 +190,7 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
         } else {
             double wsqr = 0., lsqr = 0.;
             for (int j = 0; j < L && !bad; j++) {
+                if ((j & (kRing / 2 - 1)) == 0) fill(j);
                 int c = T(j);
                 if (j > a.window - 3) {
                     const int d = T(++wb);

@@ -74,7 +74,9 @@ extern int NTHREADS;   /* report chunks (-j): fixes which pairs are skipped, so 
 extern int NSHIFT;     /* log2(NTHREADS) */
 
 /* Set_Filter_Params: replaces dalign/filter.c:119-139 (dalign/filter.h:67).
- * Returns 1 for kmer <= 1.  This implementation also rejects kmer > 14. */
+ * Returns 1 for kmer <= 1, and for kmer > 31 (undefined in the reference).
+ * A block whose keys do not fit 64 bits (2 kmer + bits of its read count
+ * + bits of its longest read > 64) is an error of Sort_Kmers. */
 int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin);
 
 /* Sort_Kmers: replaces dalign/filter.c:698-903 (dalign/filter.h:69).

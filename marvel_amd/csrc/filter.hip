@@ -1377,8 +1377,8 @@ extern "C" {
 int Set_Filter_Params(int kmer, int binshift, int suppress, int hitmin)
 {
     if (kmer <= 1) return 1;
-    if (kmer > 14) {
-        mi_set_error("k-mer length above 14 is not supported by the 64-bit k-mer keys");
+    if (kmer > 31) {   // the reference's 1 << 2k is undefined from 32 on (dalign/filter.c:129-131)
+        mi_set_error("k-mer length above 31 is not supported");
         return 1;
     }
     g_kmer = kmer;

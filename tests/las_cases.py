@@ -33,6 +33,9 @@ CASES = {
     "c1_notrace": ("small", 400, ["-T", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "c1_k12": ("small", 400, ["-k12", "-w5", "-h30", "-l500", "-e.80", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "c1_hgap": ("small", 400, ["-H6000", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    # longer k-mers (the block's 64-bit keys hold 2k + 10 read bits + 15 position bits up to k = 19)
+    "c1_k16": ("small", 400, ["-k16", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "c1_k19": ("small", 400, ["-k19", "-h50", "-j4", "-M16"], ["C1.1", "C1.1"]),
     "c1_blocks_12": ("small", 2, ["-j8", "-M16"], ["C1.1", "C1.2"]),
     "c1_blocks_21": ("small", 2, ["-j4", "-M16"], ["C1.2", "C1.1"]),
     "c1_blocks_asym": ("small", 2, ["-A", "-j8", "-M16"], ["C1.2", "C1.1", "C1.2"]),
@@ -61,8 +64,10 @@ CASES = {
     "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
     # the same with the reference DBdust track of its reads as a soft mask
     "c2_mdust": ("5mbp", 400, ["-mdust", "-j8", "-M32"], ["C2.1", "C2.1"]),
+    # the same reads in ~40 Mbp blocks: block 2 against blocks 1-4 (symmetric: the .las of both sides)
+    "c2_blocks": ("5mbp", 40, ["-j8", "-M32"], ["C2.2", "C2.1", "C2.2", "C2.3", "C2.4"]),
 }
-LARGE = {"c2_j8", "c2_mdust"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
+LARGE = {"c2_j8", "c2_mdust", "c2_blocks"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
 
 # -m tracks a case's DB carries: name -> (layout, source).  "dust" is the
 # reference DBdust output for the reads (a fixture); "rnd<seed>" random

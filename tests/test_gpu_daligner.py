@@ -188,3 +188,13 @@ def test_every_report_window_matches_reference(tmp_path, monkeypatch, level):
     finally:
         os.chdir(cwd)
     assert _shas(str(tmp_path)) == GOLD[case]["las"]
+
+
+@pytest.mark.gpu
+def test_kmer_keys_that_do_not_fit_are_an_error(tmp_path):
+    """k = 20 on the small read set needs 2k + 25 = 65 key bits: the GPU
+    daligner stops with a message instead of writing wrong output."""
+    las_cases.make_db("c1_k19", str(tmp_path))
+    r = subprocess.run([EXE, "-k20", "-j4", "-M16", "C1.1", "C1.1"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 1 and "64-bit k-mer keys" in r.stderr, r.stderr[-1000:]

@@ -43,5 +43,6 @@ def test_filter_globals_have_reference_defaults():
 def test_set_filter_params_bounds():
     lib = ctypes.CDLL(LIB)
     assert lib.Set_Filter_Params(1, 6, 0, 35) == 1     # kmer <= 1 (dalign/filter.c:121-122)
-    assert lib.Set_Filter_Params(15, 6, 0, 35) == 1    # above the 64-bit key limit
+    assert lib.Set_Filter_Params(32, 6, 0, 35) == 1    # 1 << 2k undefined from k = 32 (dalign/filter.c:129-131)
+    assert lib.Set_Filter_Params(31, 6, 0, 35) == 0    # a block's keys may still not fit: Sort_Kmers says so
     assert lib.Set_Filter_Params(14, 6, 0, 35) == 0

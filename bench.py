@@ -179,7 +179,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              "block pairs shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+              "block pairs shard across GPUs without communication (one block pair per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from marvel_amd import dbio, sim

@@ -171,8 +171,9 @@ void mi_free_kmer_index(void *table);
 
 /* Block loading for hosts without the reference's DB library: the
  * behaviour of Open_DB_Block + Read_All_Sequences (db/DB.c:360-585,
- * :1349-1410) for "DIR/ROOT.<block>" or "ROOT[.db]"; trimmed DBs and
- * .dam files are not supported.  Returns 0 or MI_ERR_ARG. */
+ * :1349-1410) for "DIR/ROOT.<block>" or "ROOT[.db|.dam]" (the stub
+ * ROOT.db, else ROOT.dam); trimmed DBs are not supported.  Returns 0 or
+ * MI_ERR_ARG. */
 int mi_open_block(const char *path, HITS_DB *db);
 void mi_close_block(HITS_DB *db);
 /* Header of the block's reverse complement for mi_sort_kmers_complement

@@ -10,8 +10,9 @@
 // 2), which decides the pairs the reference skips at chunk ends; the GPU
 // work itself is not split by it.  -m<track> (repeatable) soft-masks the
 // intervals of the named tracks, Dazzler (.anno/.data) or MARVEL
-// (.a2/.d2) layout; -b weighs k-mers by the base frequencies.  -I / -O
-// (identity) and .dam inputs are not supported.
+// (.a2/.d2) layout; -b weighs k-mers by the base frequencies; a block may be
+// of a .db or a .dam.  -I / -O (identity) are not supported.
+#include <strings.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -30,13 +31,22 @@ static double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Root(): file name without directory and ".db"
-static std::string root_of(const char* path)
+// Root(path, suffix): file name without directory and without `suffix`
+// when it ends with it (any case, db/DB.c:104-140)
+static std::string root_of(const char* path, const char* suffix)
 {
     const char* s = strrchr(path, '/');
     std::string r = s ? s + 1 : path;
-    if (r.size() > 3 && r.compare(r.size() - 3, 3, ".db") == 0) r.resize(r.size() - 3);
+    const size_t n = strlen(suffix);
+    if (r.size() > n && strcasecmp(r.c_str() + r.size() - n, suffix) == 0) r.resize(r.size() - n);
     return r;
+}
+
+// the driver's name of a block argument: without ".dam" for a .dam path, else without ".db"
+static std::string root_of(const char* path)
+{
+    const size_t n = strlen(path);
+    return root_of(path, n >= 4 && strcmp(path + n - 4, ".dam") == 0 ? ".dam" : ".db");
 }
 
 static void make_subdir(int run_id, int part)
@@ -164,7 +174,7 @@ int main(int argc, char** argv)
     if (SYMMETRIC)
         for (int i = optind + 1; i < argc; i++) {
             if (strcmp(afile, argv[i]) == 0) continue;
-            const std::string br = root_of(argv[i]);
+            const std::string br = root_of(argv[i], ".dam");   // as dalign/daligner.c:893-896 takes it
             const size_t ad = aroot.rfind('.'), bd = br.rfind('.');
             const std::string ap = aroot.substr(0, ad == std::string::npos ? aroot.size() : ad + 1);
             const std::string bp = br.substr(0, bd == std::string::npos ? br.size() : bd + 1);

@@ -8,6 +8,8 @@
 //   DIR/.ROOT.bps     : ceil(rlen/4) bytes per read, base i in bits 6-2(i%4)
 // In memory every read is one byte per base (0..3) with a 4 before and
 // after it (db/DB.c:1349-1410).
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -74,7 +76,8 @@ int mi_open_block(const char* path, HITS_DB* db)
         root = p.substr(sl + 1);
     } else
         root = p;
-    if (root.size() > 3 && root.compare(root.size() - 3, 3, ".db") == 0) root.resize(root.size() - 3);
+    if (root.size() > 4 && root.compare(root.size() - 4, 4, ".dam") == 0) root.resize(root.size() - 4);
+    else if (root.size() > 3 && root.compare(root.size() - 3, 3, ".db") == 0) root.resize(root.size() - 3);
     int part = 0;
     const size_t dot = root.rfind('.');
     if (dot != std::string::npos && dot + 1 < root.size()) {
@@ -86,7 +89,10 @@ int mi_open_block(const char* path, HITS_DB* db)
         }
     }
     int ufirst = 0, ulast = -1;
-    if (!read_stub(dir + "/" + root + ".db", part, &ufirst, &ulast)) {
+    // the stub is ROOT.db, else ROOT.dam (a DB of FA2dam; same index and bases files)
+    std::string stub = dir + "/" + root + ".db";
+    if (access(stub.c_str(), F_OK) != 0) stub = dir + "/" + root + ".dam";
+    if (!read_stub(stub, part, &ufirst, &ulast)) {
         mi::mi_set_error("cannot read the DB stub");
         return MI_ERR_ARG;
     }

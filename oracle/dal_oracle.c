@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 /* ------------------------------------------------------------------------ */
@@ -866,7 +867,8 @@ int or_block_load(or_block *b, const char *path)
         snprintf(root, sizeof(root), "%s", path);
     }
     size_t rl = strlen(root);
-    if (rl > 3 && strcmp(root + rl - 3, ".db") == 0) root[rl - 3] = 0;
+    if (rl > 4 && strcmp(root + rl - 4, ".dam") == 0) root[rl - 4] = 0;
+    else if (rl > 3 && strcmp(root + rl - 3, ".db") == 0) root[rl - 3] = 0;
     dot = strrchr(root, '.');
     if (dot && dot[1] != 0 && dot[1] != '-') {
         char *end;
@@ -876,7 +878,9 @@ int or_block_load(or_block *b, const char *path)
             root[dot - root] = 0;
         }
     }
+    /* the stub is ROOT.db, else ROOT.dam (db/DB.c:407-421) */
     snprintf(fn, sizeof(fn), "%s/%s.db", dir, root);
+    if (access(fn, F_OK) != 0) snprintf(fn, sizeof(fn), "%s/%s.dam", dir, root);
     if (read_stub_block(fn, part, &ufirst, &ulast)) return -1;
     snprintf(fn, sizeof(fn), "%s/.%s.idx", dir, root);
     idx = fopen(fn, "rb");

@@ -60,6 +60,8 @@ CASES = {
     "at_b": ("at_rich", 400, ["-b", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "at_b_mrnd": ("at_rich", 400, ["-b", "-mrnd", "-j4", "-M16"], ["C1.1", "C1.1"]),
     "at_b_blocks": ("at_rich", 1, ["-b", "-j8", "-M16"], ["C1.3", "C1.1", "C1.2"]),
+    # a .dam database (the stub named ROOT.dam, as FA2dam writes it), two blocks
+    "c1_dam": ("small", 2, ["-j8", "-M16"], ["C1.2", "C1.1"]),
     # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)
     "c2_j8": ("5mbp", 400, ["-j8", "-M32"], ["C2.1", "C2.1"]),
     # the same with the reference DBdust track of its reads as a soft mask
@@ -81,6 +83,7 @@ MASKS = {
     "c2_mdust": {"dust": ("dazzler", "dust")},
     "at_b_mrnd": {"rnd": ("dazzler", "rnd9")},
 }
+DAM = {"c1_dam"}   # cases whose DB stub is ROOT.dam
 MASK_FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mask")
 
 _reads = {}
@@ -131,6 +134,8 @@ def make_db(case: str, d: str) -> None:
     root = os.path.join(d, blocks[0].split(".")[0])
     reads = reads_of(data)
     dbio.write_db(root, reads, size_mbp=split)
+    if case in DAM:
+        os.replace(root + ".db", root + ".dam")
     lens = [len(r) for r in reads]
     for name, (fmt, src) in MASKS.get(case, {}).items():
         if src == "dust":

@@ -2,7 +2,7 @@
 // dalign/daligner.c:615-1032 over libmarvl_align's Sort_Kmers /
 // Match_Filter / Write_Overlap_Buffer.
 //
-//   daligner [-vbAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]
+//   daligner [-vbAITO] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]
 //            [-e<double(.70)>] [-l<int(1000)>] [-s<int(100)>] [-H<int>]
 //            [-r<int(1)>] [-j<int(4)>] [-m<track>]+ <subject:db> <target:db> ...
 //
@@ -10,8 +10,9 @@
 // 2), which decides the pairs the reference skips at chunk ends; the GPU
 // work itself is not split by it.  -m<track> (repeatable) soft-masks the
 // intervals of the named tracks, Dazzler (.anno/.data) or MARVEL
-// (.a2/.d2) layout; -b weighs k-mers by the base frequencies; a block may be
-// of a .db or a .dam.  -I / -O (identity) are not supported.
+// (.a2/.d2) layout; -b weighs k-mers by the base frequencies; -I also
+// pairs every read with itself, -O keeps only those overlaps; a block may be
+// of a .db or a .dam.
 #include <strings.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -60,7 +61,7 @@ static void make_subdir(int run_id, int part)
 static void usage()
 {
     fprintf(stderr,
-            "usage: daligner [-vbAT] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]\n"
+            "usage: daligner [-vbAITO] [-k<int(14)>] [-w<int(6)>] [-h<int(35)>] [-t<int>] [-M<int>]\n"
             "                [-e<double(.70)>] [-l<int(1000)>] [-s<int(100)>] [-H<int>] [-r<int(1)>] [-j<int(4)>]\n"
             "                [-m<track>]+\n"
             "                <subject:db> <target:db> ...\n");
@@ -84,7 +85,9 @@ int main(int argc, char** argv)
     int kmer = 14, binshift = 6, hitmin = 35, suppress = 0, tspace = 100, run_id = 1, no_trace = 0, l = 1000;
     double ave_err = .70;
     std::vector<const char*> masks;
+    int only_identity = 0;
     VERBOSE = 0;
+    IDENTITY = 0;
     SYMMETRIC = 1;
     HGAP_MIN = 0;
     NTHREADS = 4;
@@ -96,6 +99,11 @@ int main(int argc, char** argv)
         switch (c) {
         case 'v': VERBOSE += 1; break;
         case 'b': BIASED = 1; break;
+        case 'I': IDENTITY = 1; break;
+        case 'O':
+            IDENTITY = 1;
+            only_identity = 1;
+            break;
         case 'T': no_trace = 1; break;
         case 'A': SYMMETRIC = 0; break;
         case 'k': kmer = atoi(optarg); break;
@@ -184,7 +192,7 @@ int main(int argc, char** argv)
             }
         }
     make_subdir(run_id, A.part);
-    Align_Spec* spec = New_Align_Spec(ave_err, tspace, A.freq, NTHREADS, SYMMETRIC, 0, no_trace);
+    Align_Spec* spec = New_Align_Spec(ave_err, tspace, A.freq, NTHREADS, SYMMETRIC, only_identity, no_trace);
     char* aname = strdup(aroot.c_str());
     void* aindex = nullptr;
     for (int i = optind + 1; i < argc; i++) {

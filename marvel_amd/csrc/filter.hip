@@ -835,8 +835,8 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     hist.ensure(MAXGRAM);
     HC(hipMemsetAsync(hist.p, 0, sizeof(u64) * MAXGRAM, st));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_brange), dim3(nblk(ng)), dim3(256), 0, st, A->keys.p, A->fmt, A->gstart.p, ng, B->keys.p, B->fmt, nb, same_index ? 1 : 0, bs.p, be.p);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, be.p, B->keys.p, B->fmt, self ? 1 : 0,
-                                       cnt.p);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, be.p, B->keys.p, B->fmt,
+                       self ? (IDENTITY ? (comp ? 3 : 2) : 1) : 0, cnt.p);
     g_scan.run(cnt.p, off.p, na);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_totals), dim3(nblk(ng, 256 * 16)), dim3(256), 0, st, A->gstart.p, ng, off.p, gtot.p, hist.p, MAXGRAM);
     HC(hipGetLastError());
@@ -1468,7 +1468,6 @@ int mi_match_filter(char* aname, HITS_DB* ablock, char* bname, HITS_DB* bblock, 
         KIdx* A = as_index(atable);
         KIdx* B = as_index(btable);
         const bool self = (aname == bname);
-        if (IDENTITY) throw Fail("identity mode (-I) is not supported", MI_ERR_ARG);
         match_pass(A, B, self, comp ? 1 : 0, atable == btable, (SpecObj*)asettings);
     } catch (const Fail& e) {
         mi_set_error(e.what());

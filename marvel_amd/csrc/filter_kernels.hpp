@@ -277,12 +277,18 @@ __global__ void k_counts(const uint64_t* __restrict__ ak, KeyFmt fa, long long n
     if (i >= na) return;
     const long long g = gidx[i];
     long long s = bs[g], e = be[g];
+    // self: B entries of reads below A's (1); with -I also A's own read, at
+    // positions below A's (2) or, against the complement, wholly (3)
+    // (dalign/filter.c:984-1001)
     if (self) {
-        const int ar = fa.read(ak[i]);
+        const uint64_t ka = ak[i];
+        const int ar = fa.read(ka), ap = fa.pos(ka);
         long long lo = s, hi = e;
         while (lo < hi) {
             const long long m = (lo + hi) >> 1;
-            if (fb.read(bk[m]) < ar) lo = m + 1;
+            const int br = fb.read(bk[m]);
+            const bool below = self == 3 ? br <= ar : (br < ar || (self == 2 && br == ar && fb.pos(bk[m]) < ap));
+            if (below) lo = m + 1;
             else hi = m;
         }
         e = lo;

@@ -395,9 +395,14 @@ __global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report
                             continue;
                         }
                     }
+                    // a read against itself (-I, self forward pass: the same bases): the
+                    // wave may not reach the main diagonal (Local_Alignment's selfie
+                    // bounds, dalign/align.c:1966-1985)
+                    const bool selfie = a.self && !a.comp && ar == br;
                     LaResult r = local_alignment<NCH>(a.sp, Aseq, alen, Bseq, blen, a.comp, dg, dg, ap + bp,
-                                                      -INT32_MAX, INT32_MAX, pool, a.pool_cap, pts, a.pts_cap, atr,
-                                                      btr, maxtp, a.abort_flag, wc);
+                                                      (selfie && dg >= 0) ? 1 : -INT32_MAX,
+                                                      (selfie && dg <= 0) ? -1 : INT32_MAX, pool, a.pool_cap, pts,
+                                                      a.pts_cap, atr, btr, maxtp, a.abort_flag, wc);
                     if (r.status != LA_OK) {
                         pstatus = (r.status == LA_WIDE || r.status == LA_POOL || r.status == LA_ABORT) ? RP_RETRY
                                                                                                         : RP_UNDEF;

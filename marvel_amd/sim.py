@@ -152,6 +152,31 @@ def config_at_rich(seed: int = 31):
     return simulate_reads(g, 20.0, 4000.0, 2000.0, 500, seed + 1000)
 
 
+def config_self_repeats(seed: int = 41):
+    """config_small's read model with repeats inside reads: 30 % of the reads
+    (>= 8 kb) hold a copy of one of their own 1-2.5 kb segments (4 % point
+    changes) at least 3 kb further on, 15 % an inverted copy (reverse
+    complement); the seeds daligner -I pairs a read with itself on."""
+    rs = config_small(seed)
+    rng = np.random.default_rng(seed + 7)
+    out = []
+    for r in rs.reads:
+        r = r.copy()
+        u = rng.random()
+        if len(r) >= 8000 and u < 0.45:
+            ln = int(rng.integers(1000, 2500))
+            s0 = int(rng.integers(0, len(r) - ln - 3000 - ln))
+            d0 = int(rng.integers(s0 + ln + 3000, len(r) - ln + 1))
+            seg = r[s0:s0 + ln].copy()
+            if u >= 0.30:
+                seg = (3 - seg[::-1]).astype(np.uint8)
+            flip = rng.random(ln) < 0.04
+            seg[flip] = (seg[flip] + rng.integers(1, 4, size=int(flip.sum()))) & 3
+            r[d0:d0 + ln] = seg
+        out.append(r)
+    return ReadSet(out, rs.starts, rs.lens_t, rs.strand)
+
+
 def config_band_edges(seed: int = 21):
     """Reads that drive report_thread's outermost diagonal bands: two reads of
     the block's maximum length X1, X2 (4,990 bp); Y begins with X1's last 60 bp

@@ -1411,6 +1411,17 @@ typedef struct {
 
 #define MAXGRAM 10000
 
+/* B entries of a code group that pair with A entry a in self mode
+ * (dalign/filter.c:984-1001, :1101-1146): reads below a's, and with -I
+ * also a's own read -- at positions below a's, or wholly against the
+ * complement.  mode 1: self; 2: self -I; 3: self -I, complement. */
+static inline int pairs_below(int mode, const kmer_t *b, const kmer_t *a)
+{
+    if (mode == 3) return b->read <= a->read;
+    if (mode == 2) return b->read < a->read || (b->read == a->read && b->rpos < a->rpos);
+    return b->read < a->read;
+}
+
 static seed_t *make_seeds(const or_opts *o, kmer_t *as, int64_t na, kmer_t *bs, int64_t nb,
                           int self, int same_index, int64_t *nseed, int *limit_out)
 {
@@ -1427,7 +1438,7 @@ static seed_t *make_seeds(const or_opts *o, kmer_t *as, int64_t na, kmer_t *bs, 
         if (self) {
             int64_t q = jb;
             for (int64_t a = ja; a < ia; a++) {
-                while (q < ib && bs[q].read < as[a].read) q++;
+                while (q < ib && pairs_below(self, &bs[q], &as[a])) q++;
                 ct += q - jb;
             }
         } else
@@ -1475,7 +1486,7 @@ static seed_t *make_seeds(const or_opts *o, kmer_t *as, int64_t na, kmer_t *bs, 
         if (self) {
             int64_t q = jb;
             for (int64_t a = ja; a < ia; a++) {
-                while (q < ib && bs[q].read < as[a].read) q++;
+                while (q < ib && pairs_below(self, &bs[q], &as[a])) q++;
                 ct += q - jb;
             }
         } else
@@ -1485,7 +1496,7 @@ static seed_t *make_seeds(const or_opts *o, kmer_t *as, int64_t na, kmer_t *bs, 
         for (int64_t a = ja; a < ia; a++) {
             int64_t e = ib;
             if (self) {
-                while (q < ib && bs[q].read < as[a].read) q++;
+                while (q < ib && pairs_below(self, &bs[q], &as[a])) q++;
                 e = q;
             }
             for (int64_t bb = jb; bb < e; bb++) {
@@ -1812,7 +1823,8 @@ static int or_daligner_pass(const or_opts *o, const or_spec *sp, or_block *A, or
     int64_t nh;
     int limit;
     if (na == 0 || nb == 0) return 0;
-    seed_t *hits = make_seeds(o, as, na, bs, nb, self, same_index, &nh, &limit);
+    seed_t *hits = make_seeds(o, as, na, bs, nb, self ? (o->identity ? (comp ? 3 : 2) : 1) : 0, same_index, &nh,
+                              &limit);
     st->nhits += nh;
     st->limit = limit;
     if (nh == 0) { free(hits); return 0; }

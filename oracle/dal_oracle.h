@@ -89,6 +89,7 @@ typedef struct {
     int kmer, binshift, hitmin, suppress, minover, tspace, nthreads, nshift;
     int hgap_min, symmetric, run_id, no_trace, verbose;
     int biased;   /* -b: k-mers of variable length weighted by the base frequencies */
+    int identity, only_identity;   /* -I: seeds of a read with itself too; -O: keep only those overlaps */
     double ave_corr;
     uint64_t mem_limit;
 } or_opts;

@@ -62,6 +62,8 @@ int main(int argc, char **argv)
         switch (c) {
         case 'v': o.verbose += 1; break;
         case 'b': o.biased = 1; break;
+        case 'I': o.identity = 1; break;
+        case 'O': o.identity = 1; o.only_identity = 1; break;
         case 'T': o.no_trace = 1; break;
         case 'A': o.symmetric = 0; break;
         case 'k': o.kmer = atoi(optarg); break;
@@ -160,6 +162,12 @@ int main(int argc, char **argv)
         }
         double t0 = now();
         or_daligner_pair(&o, sp, &A, self ? &A : &B, self, &out, &st);
+        if (o.only_identity) {   /* -O: only a read's overlaps with itself are written (dalign/align.c:5786-5800) */
+            int64_t k = 0;
+            for (int64_t j = 0; j < out.n; j++)
+                if (out.ovl[j].aread == out.ovl[j].bread) out.ovl[k++] = out.ovl[j];
+            out.n = k;
+        }
         or_ovlset_sort(&out);
         double t1 = now();
         char p1[9000], p2[9000], d1[64], d2[64];

@@ -21,6 +21,7 @@ DATA = {
     "5mbp": lambda: sim.config_5mbp(2),
     "edges": lambda: sim.config_band_edges(21),
     "at_rich": lambda: sim.config_at_rich(31),
+    "self_rep": lambda: sim.config_self_repeats(41),
 }
 
 # name -> (data, DBsplit size in Mbp, daligner flags, block arguments)
@@ -60,6 +61,13 @@ CASES = {
     "at_b": ("at_rich", 400, ["-b", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "at_b_mrnd": ("at_rich", 400, ["-b", "-mrnd", "-j4", "-M16"], ["C1.1", "C1.1"]),
     "at_b_blocks": ("at_rich", 1, ["-b", "-j8", "-M16"], ["C1.3", "C1.1", "C1.2"]),
+    # identity mode on reads holding copies of their own segments (direct and inverted): each read against
+    # itself too (-I), only those overlaps (-O), asymmetric, over blocks.  (The reference's -I crashes on the
+    # tandem-repeat reads of small_rep.)
+    "srep_I": ("self_rep", 400, ["-I", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "srep_O": ("self_rep", 400, ["-O", "-j4", "-M16"], ["C1.1", "C1.1"]),
+    "srep_I_A": ("self_rep", 400, ["-I", "-A", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "srep_I_blocks": ("self_rep", 2, ["-I", "-j8", "-M16"], ["C1.2", "C1.1", "C1.2"]),
     # a .dam database (the stub named ROOT.dam, as FA2dam writes it), two blocks
     "c1_dam": ("small", 2, ["-j8", "-M16"], ["C1.2", "C1.1"]),
     # the 5 Mbp / 30x workload (one ~158 Mbp block against itself)

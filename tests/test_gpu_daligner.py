@@ -92,7 +92,8 @@ INTEG = os.path.join(ROOT, "oracle", "_ref", "daligner_on_libmarvl")
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(INTEG), reason="integration binary not built (make -C oracle integ)")
-@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "rep_mdust_mrnd", "rep_mask_blocks", "at_b_mrnd", "c2_j8"])
+@pytest.mark.parametrize("case", ["c1_j8", "c1_blocks_12", "rep_mdust_mrnd", "rep_mask_blocks", "at_b_mrnd", "srep_O",
+                                  "c2_j8"])
 def test_reference_driver_on_library(case):
     """The reference's own daligner.c and DB code, linked against
     libmarvl_align instead of its filter.c / align.c, writes the golden .las.

@@ -100,7 +100,10 @@ class Daligner:
         self.h = h
 
     # ---- driver globals (dalign/daligner.c:60-71) ----
-    def configure(self, nthreads=4, mem_limit_gb=None, minover=2000, symmetric=1, hgap_min=0, verbose=0, biased=0):
+    def configure(self, nthreads=4, mem_limit_gb=None, minover=2000, symmetric=1, hgap_min=0, verbose=0, biased=0,
+                  identity=0):
+        """The driver's globals (dalign/daligner.c:60-71): -j, -M, -l, -A, -H, -v, -b, -I.  -O is
+        new_align_spec(..., only_identity=1) with identity=1."""
         if nthreads < 1 or nthreads & (nthreads - 1):
             raise DalignerError("nthreads must be a power of 2")
         C.c_int.in_dll(self.h, "NTHREADS").value = nthreads
@@ -112,6 +115,7 @@ class Daligner:
         C.c_int.in_dll(self.h, "HGAP_MIN").value = hgap_min
         C.c_int.in_dll(self.h, "VERBOSE").value = verbose
         C.c_int.in_dll(self.h, "BIASED").value = biased
+        C.c_int.in_dll(self.h, "IDENTITY").value = identity
 
     def _err(self, what):
         raise DalignerError(f"{what}: {self.h.mi_last_error().decode(errors='replace')}")

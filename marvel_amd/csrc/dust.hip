@@ -1,9 +1,9 @@
 // DBdust on the GPU: the SDUST low-complexity scan of db/DBdust.c:236-470
 // (Morgulis et al., JCB 13(5) 2006) over every read of a block, one lane per
 // read.  The scan is sequential along a read, so the parallelism is across
-// reads (16 lanes per workgroup, longest reads first); each lane keeps its
-// triplet counts and (for windows up to 64) its candidate list in LDS
-// ([item][lane]), its interval stack in global memory.
+// reads; each lane keeps its triplet counts and (for windows up to 64) its
+// candidate list in LDS ([item][lane], a bank per lane), its interval stack
+// in global memory.
 //
 // Per read (triplet codes t[0 .. len-3] of bases j, j+1, j+2):
 //   * the window counts cover (wb, j], the scanned run (lb, j]; wsqr / lsqr
@@ -20,7 +20,6 @@
 // Score arithmetic is the reference's: doubles, no fused multiply-add.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -51,7 +50,6 @@ struct DustArgs {
     long long nslots;
     int* stack;                  // per read: the interval stack at soff[r] (rlen[r] + 4 ints)
     const long long* soff;
-    const int* order;            // the reads by decreasing length
     int* npts;                   // out: points kept per read (written to stack + soff[r])
     int* status;                 // out: nonzero when a read ran out of candidate nodes
 };
@@ -60,15 +58,12 @@ struct DustArgs {
 // (the default 64): every step of the scan touches them, and HBM latency
 // would serialise each lane's walk.  Larger windows keep them in HBM.
 constexpr int kDustLdsWindow = 64;
-// lanes (reads) per workgroup: few, so that a wave's reads take few distinct
-// paths per step and several workgroups share a CU
-constexpr int kDustLanes = 16;
 
 template <bool LDS_NODES>
-__global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
+__global__ __launch_bounds__(64) void k_sdust(DustArgs a)
 {
-    __shared__ int wcnt[64 * kDustLanes], lcnt[64 * kDustLanes];   // [triplet][lane]
-    constexpr int kN = LDS_NODES ? (kDustLdsWindow + 1) * kDustLanes : 1;
+    __shared__ int wcnt[64 * 64], lcnt[64 * 64];   // [triplet][lane]
+    constexpr int kN = LDS_NODES ? (kDustLdsWindow + 1) * 64 : 1;
     __shared__ int sbeg[kN], send[kN];
     __shared__ double ssc[kN];
     __shared__ short snx[kN], spv[kN];
@@ -77,28 +72,27 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
     // scan reads only positions in (j - window, j], so a slot is rewritten
     // only after its position left the window.
     constexpr int kRing = LDS_NODES ? 128 : 2 * kDustMaxWindow;
-    __shared__ uint8_t ring[kRing * kDustLanes];
+    __shared__ uint8_t ring[kRing * 64];
     const int lane = threadIdx.x;
-    const long long slot = (long long)blockIdx.x * kDustLanes + lane;
+    const long long slot = (long long)blockIdx.x * 64 + lane;
     const long long S = a.nslots;
-    for (long long s = slot; s < a.nreads; s += S) {
-        const int r = a.order[s];   // reads longest first: a workgroup's reads end together
-        for (int e = 0; e < 64; e++) wcnt[e * kDustLanes + lane] = lcnt[e * kDustLanes + lane] = 0;
+    for (int r = (int)slot; r < a.nreads; r += (int)S) {
+        for (int e = 0; e < 64; e++) wcnt[e * 64 + lane] = lcnt[e * 64 + lane] = 0;
         const uint8_t* b = a.bases + a.boff[r];
         const int L = a.rlen[r] - 2;
-        auto T = [&](int i) { return (int)ring[(i & (kRing - 1)) * kDustLanes + lane]; };
+        auto T = [&](int i) { return (int)ring[(i & (kRing - 1)) * 64 + lane]; };
         auto fill = [&](int j0) {   // codes of positions [j0, j0 + kRing/2) that exist
             const int j1 = min(j0 + kRing / 2, L);
 #pragma unroll 8
             for (int i = j0; i < j1; i++)
-                ring[(i & (kRing - 1)) * kDustLanes + lane] = (uint8_t)((b[i] << 4) | (b[i + 1] << 2) | b[i + 2]);
+                ring[(i & (kRing - 1)) * 64 + lane] = (uint8_t)((b[i] << 4) | (b[i + 1] << 2) | b[i + 2]);
         };
         // candidate nodes: 0 is the list head (beg = -2), 1..window the free list
-        auto BEG = [&](int n) -> int& { return LDS_NODES ? sbeg[n * kDustLanes + lane] : a.cbeg[n * S + slot]; };
-        auto END = [&](int n) -> int& { return LDS_NODES ? send[n * kDustLanes + lane] : a.cend[n * S + slot]; };
-        auto SC = [&](int n) -> double& { return LDS_NODES ? ssc[n * kDustLanes + lane] : a.cscore[n * S + slot]; };
-        auto NX = [&](int n) -> short& { return LDS_NODES ? snx[n * kDustLanes + lane] : a.cnext[n * S + slot]; };
-        auto PV = [&](int n) -> short& { return LDS_NODES ? spv[n * kDustLanes + lane] : a.cprev[n * S + slot]; };
+        auto BEG = [&](int n) -> int& { return LDS_NODES ? sbeg[n * 64 + lane] : a.cbeg[n * S + slot]; };
+        auto END = [&](int n) -> int& { return LDS_NODES ? send[n * 64 + lane] : a.cend[n * S + slot]; };
+        auto SC = [&](int n) -> double& { return LDS_NODES ? ssc[n * 64 + lane] : a.cscore[n * S + slot]; };
+        auto NX = [&](int n) -> short& { return LDS_NODES ? snx[n * 64 + lane] : a.cnext[n * S + slot]; };
+        auto PV = [&](int n) -> short& { return LDS_NODES ? spv[n * 64 + lane] : a.cprev[n * S + slot]; };
         NX(0) = 0;
         PV(0) = 0;
         BEG(0) = -2;
@@ -131,19 +125,19 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
                 int c = T(j);
                 if (j > a.window - 3) {
                     const int d = T(++wb);
-                    wsqr -= --wcnt[d * kDustLanes + lane];
+                    wsqr -= --wcnt[d * 64 + lane];
                 }
-                wsqr += wcnt[c * kDustLanes + lane]++;
+                wsqr += wcnt[c * 64 + lane]++;
                 if (lb < wb) {
                     const int d = T(++lb);
-                    lsqr -= --lcnt[d * kDustLanes + lane];
+                    lsqr -= --lcnt[d * 64 + lane];
                 }
-                const int trun = lcnt[c * kDustLanes + lane]++;
+                const int trun = lcnt[c * 64 + lane]++;
                 lsqr += trun;
                 if (trun >= a.thresh2i)
                     while (lb < j) {
                         const int d = T(++lb);
-                        lsqr -= --lcnt[d * kDustLanes + lane];
+                        lsqr -= --lcnt[d * 64 + lane];
                         if (d == c) break;
                     }
                 int J = PV(0);
@@ -154,7 +148,7 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
                 double mscore = 0.;
                 for (c = lb; c > wb; c--) {
                     const int d = T(c);
-                    lsqr += lcnt[d * kDustLanes + lane]++;
+                    lsqr += lcnt[d * 64 + lane]++;
                     if (lsqr >= a.thresh * (j - c)) {
                         for (int g = 0; BEG(J) >= c; Lp = J, J = NX(J)) {
                             if (++g > a.window + 1) {   // a list longer than its nodes: stop the read
@@ -190,7 +184,7 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
                 }
                 for (c++; c <= lb; c++) {
                     const int d = T(c);
-                    lsqr -= --lcnt[d * kDustLanes + lane];
+                    lsqr -= --lcnt[d * 64 + lane];
                 }
             }
         } else {
@@ -200,19 +194,19 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
                 int c = T(j);
                 if (j > a.window - 3) {
                     const int d = T(++wb);
-                    wsqr -= (--wcnt[d * kDustLanes + lane]) * a.skew[d];
+                    wsqr -= (--wcnt[d * 64 + lane]) * a.skew[d];
                 }
-                wsqr += (wcnt[c * kDustLanes + lane]++) * a.skew[c];
+                wsqr += (wcnt[c * 64 + lane]++) * a.skew[c];
                 if (lb < wb) {
                     const int d = T(++lb);
-                    lsqr -= (--lcnt[d * kDustLanes + lane]) * a.skew[d];
+                    lsqr -= (--lcnt[d * 64 + lane]) * a.skew[d];
                 }
-                const double trun = (lcnt[c * kDustLanes + lane]++) * a.skew[c];
+                const double trun = (lcnt[c * 64 + lane]++) * a.skew[c];
                 lsqr += trun;
                 if (trun >= a.thresh2r)
                     while (lb < j) {
                         const int d = T(++lb);
-                        lsqr -= (--lcnt[d * kDustLanes + lane]) * a.skew[d];
+                        lsqr -= (--lcnt[d * 64 + lane]) * a.skew[d];
                         if (d == c) break;
                     }
                 int J = PV(0);
@@ -223,7 +217,7 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
                 double mscore = 0.;
                 for (c = lb; c > wb; c--) {
                     const int d = T(c);
-                    lsqr += (lcnt[d * kDustLanes + lane]++) * a.skew[d];
+                    lsqr += (lcnt[d * 64 + lane]++) * a.skew[d];
                     if (lsqr >= a.thresh * (j - c)) {
                         for (int g = 0; BEG(J) >= c; Lp = J, J = NX(J)) {
                             if (++g > a.window + 1) {   // a list longer than its nodes: stop the read
@@ -259,7 +253,7 @@ __global__ __launch_bounds__(kDustLanes) void k_sdust(DustArgs a)
                 }
                 for (c++; c <= lb; c++) {
                     const int d = T(c);
-                    lsqr -= (--lcnt[d * kDustLanes + lane]) * a.skew[d];
+                    lsqr -= (--lcnt[d * 64 + lane]) * a.skew[d];
                 }
             }
         }
@@ -330,22 +324,18 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
                 skew[p++] = .015625 / f;
             }
     const int cus = std::max(1, mi_num_cus());
-    const int grid = (int)std::min<long long>((n + kDustLanes - 1) / kDustLanes, (long long)cus * 32);
-    const long long nslots = (long long)grid * kDustLanes;
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return rlen[x] > rlen[y]; });
+    const int grid = (int)std::min<long long>((n + 63) / 64, (long long)cus * 8);
+    const long long nslots = (long long)grid * 64;
     const bool lds_nodes = window <= kDustLdsWindow;
     const long long nodes = lds_nodes ? 1 : (long long)(window + 1) * nslots;
     void *dbases = nullptr, *dboff = nullptr, *drlen = nullptr, *dskew = nullptr, *dbeg = nullptr, *dend = nullptr,
          *dsc = nullptr, *dnx = nullptr, *dpv = nullptr, *dstk = nullptr, *dsoff = nullptr, *dnp = nullptr,
-         *dst = nullptr, *dord = nullptr;
+         *dst = nullptr;
     int rc = 0;
     auto M = [&](void** p, size_t b) { return rc == 0 && hipMalloc(p, std::max<size_t>(b, 8)) == hipSuccess; };
     if (!M(&dbases, hi - lo) || !M(&dboff, 8 * n) || !M(&drlen, 4 * n) || !M(&dskew, sizeof skew) ||
         !M(&dbeg, 4 * nodes) || !M(&dend, 4 * nodes) || !M(&dsc, 8 * nodes) || !M(&dnx, 2 * nodes) ||
-        !M(&dpv, 2 * nodes) || !M(&dstk, 4 * tot) || !M(&dsoff, 8 * (n + 1)) || !M(&dnp, 4 * n) || !M(&dst, 4 * n) ||
-        !M(&dord, 4 * n)) {
+        !M(&dpv, 2 * nodes) || !M(&dstk, 4 * tot) || !M(&dsoff, 8 * (n + 1)) || !M(&dnp, 4 * n) || !M(&dst, 4 * n)) {
         mi_set_error("mi_dust: device allocation failed");
         rc = MI_ERR_DEVICE;
     }
@@ -371,7 +361,6 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
         a.nslots = nslots;
         a.stack = (int*)dstk;
         a.soff = (const long long*)dsoff;
-        a.order = (const int*)dord;
         a.npts = (int*)dnp;
         a.status = (int*)dst;
         const bool ok =
@@ -379,11 +368,10 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
             hipMemcpy(dboff, boff.data(), 8 * n, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(drlen, rlen.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(dskew, skew, sizeof skew, hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dsoff, soff.data(), 8 * (n + 1), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(dord, order.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess;
+            hipMemcpy(dsoff, soff.data(), 8 * (n + 1), hipMemcpyHostToDevice) == hipSuccess;
         if (ok) {
-            if (lds_nodes) hipLaunchKernelGGL(k_sdust<true>, dim3(grid), dim3(kDustLanes), 0, 0, a);
-            else hipLaunchKernelGGL(k_sdust<false>, dim3(grid), dim3(kDustLanes), 0, 0, a);
+            if (lds_nodes) hipLaunchKernelGGL(k_sdust<true>, dim3(grid), dim3(64), 0, 0, a);
+            else hipLaunchKernelGGL(k_sdust<false>, dim3(grid), dim3(64), 0, 0, a);
         }
         stk.resize(tot);
         if (!ok || hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
@@ -394,7 +382,7 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
             rc = MI_ERR_DEVICE;
         }
     }
-    for (void* p : {dbases, dboff, drlen, dskew, dbeg, dend, dsc, dnx, dpv, dstk, dsoff, dnp, dst, dord})
+    for (void* p : {dbases, dboff, drlen, dskew, dbeg, dend, dsc, dnx, dpv, dstk, dsoff, dnp, dst})
         if (p) (void)hipFree(p);
     if (rc) return rc;
     long long o = 0;

@@ -23,7 +23,7 @@ def la_batch(ts, return_counters: bool = False):
         t = tasks[i]
         t.aword, t.bword = int(aw[i]), int(bw[i])
         t.alen, t.blen, t.comp, t.low, t.hgh, t.anti, t.lbord, t.hbord = (int(x) for x in p)
-        t.selfie = 0
+        t.selfie = int(ts.selfie[i]) if ts.selfie is not None else 0
     res = (MiLaResult * n)()
     cap = int(sum(4 * (max(int(p[0]), int(p[1])) // ts.tspace + 2) for p in ts.par)) + 16
     tr = np.zeros(cap, dtype=np.uint16)

@@ -82,18 +82,26 @@ class TaskSet:
     freq: tuple
     reads: list               # [a0, b0, a1, b1, ...]
     par: np.ndarray           # int32 [n, 8]: alen, blen, comp, low, hgh, anti, lbord, hbord
+    selfie: np.ndarray = None  # bool [n]: B is A itself (same bases), Local_Alignment's selfie case
 
     @property
     def n(self) -> int:
         return self.par.shape[0]
 
+    def _self_offsets(self, ao, bo):
+        if self.selfie is not None:
+            bo = np.where(self.selfie, ao, bo)
+        return ao, bo
+
     def blob(self):
         seq, offs = seq_blob(self.reads)
-        return seq, offs[0::2].copy(), offs[1::2].copy()
+        ao, bo = self._self_offsets(offs[0::2].copy(), offs[1::2].copy())
+        return seq, ao, bo
 
     def packed(self):
         w, offs = pack_reads(self.reads)
-        return w, offs[0::2].copy(), offs[1::2].copy()
+        ao, bo = self._self_offsets(offs[0::2].copy(), offs[1::2].copy())
+        return w, ao, bo
 
     def task_file_bytes(self) -> bytes:
         """The oracle/la_vectors task-file encoding."""
@@ -132,6 +140,42 @@ def make_tasks(n: int, seed: int, lmin: int = 1000, lmax: int = 60000, emin: flo
         reads += [a, b]
         par[i] = (len(a), len(b), comp, apos - bpos, apos - bpos, apos + bpos, -1, -1)
     return TaskSet(ave_corr, tspace, tuple(freq), reads, par)
+
+
+def make_self_tasks(n: int, seed: int, lmin: int = 3000, lmax: int = 20000, emin: float = 0.02,
+                    emax: float = 0.15, tandem_frac: float = 0.5) -> TaskSet:
+    """n tasks aligning a read against itself (B = A, the same bases: the
+    reference's selfie case, dalign/align.c:1966-1985).  Each read holds a
+    mutated copy (error e ~ U[emin, emax]) of one of its own segments
+    further on: far away, or (tandem_frac of the tasks) right after a 20-300
+    bp segment, whose wave comes near the main diagonal the selfie bounds
+    keep it off.  The seed sits on the copy's diagonal, above the main
+    diagonal for half of the tasks and below it (A and B roles swapped) for
+    the others."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reads = []
+    par = np.zeros((n, 8), dtype=np.int32)
+    for i in range(n):
+        L = int(math.exp(rng.uniform(math.log(lmin), math.log(lmax))))
+        t = rng.integers(0, 4, size=L, dtype=np.uint8)
+        if rng.random() < tandem_frac:
+            ln = int(rng.integers(20, 300))
+            s0 = int(rng.integers(0, L - 3 * ln))
+            d0 = s0 + ln
+        else:
+            ln = int(rng.integers(L // 8, L // 3))
+            s0 = int(rng.integers(0, L - 2 * ln - 100))
+            d0 = int(rng.integers(s0 + ln + 50, L - ln))
+        e = rng.uniform(emin, emax)
+        copy, cmap = mutate_map(t[s0:s0 + ln], rng, e * 9 / 15, e * 4 / 15, e * 2 / 15)
+        a = np.concatenate([t[:d0], copy, t[d0 + ln:]]).astype(np.uint8)
+        x = int(rng.integers(ln // 4, 3 * ln // 4))
+        apos, bpos = d0 + int(cmap[x]), s0 + x
+        if rng.random() < 0.5:
+            apos, bpos = bpos, apos
+        reads += [a, a]
+        par[i] = (len(a), len(a), 0, apos - bpos, apos - bpos, apos + bpos, -1, -1)
+    return TaskSet(0.70, 100, (0.25, 0.25, 0.25, 0.25), reads, par, np.ones(n, dtype=bool))
 
 
 def parse_results(buf: bytes, n: int):

@@ -43,11 +43,15 @@ LA_SETS = {
     "biased": dict(n=200, seed=106, lmin=1000, lmax=10000, offmax=3, freq=(0.35, 0.15, 0.15, 0.35)),
     # very short reads: seeds close to both ends, clipping paths
     "short": dict(n=400, seed=107, lmin=40, lmax=400, emin=0.0, emax=0.25, offmax=30),
+    # a read against itself (the same bases): the selfie bounds keep the wave off the main diagonal
+    "selfie": dict(self=True, n=400, seed=108),
 }
 
 
 def la_set(name):
     kw = dict(LA_SETS[name])
+    if kw.pop("self", False):
+        return latasks.make_self_tasks(**kw)
     return latasks.make_tasks(**kw)
 
 

@@ -19,7 +19,7 @@ def load(name):
     kw = {k: meta[k] for k in meta if k not in ("task_sha256", "producer")}
     if "freq" in kw:
         kw["freq"] = tuple(kw["freq"])
-    ts = latasks.make_tasks(**kw)
+    ts = latasks.make_self_tasks(**kw) if kw.pop("self", False) else latasks.make_tasks(**kw)
     assert ts.digest() == meta["task_sha256"], "task generator no longer reproduces the golden inputs"
     res = latasks.parse_results(gzip.open(os.path.join(GOLD, name + ".res.gz")).read(), ts.n)
     return ts, res

@@ -170,6 +170,44 @@ typedef struct {
 int mi_la_batch_host(const MiAlignParams *prm, const uint64_t *packed, long long npacked, const MiLaTask *tasks,
                      int ntasks, MiLaResult *results, uint16_t *traces, long long ntrace);
 
+/* The same batch on device-resident data (no reference counterpart: the
+ * batched entry for callers that keep reads and tasks in HBM).  d_packed,
+ * d_tasks, d_order (optional queue order, NULL = task order), d_results,
+ * d_traces and d_trace_used are device pointers.  Each completed task's A
+ * trace and then its B trace are appended to the trace arena d_traces
+ * (capacity trace_cap uint16 values) at the offset *d_trace_used held, which
+ * the call advances; results[i].atrace / btrace are arena offsets.  A task
+ * that does not fit gets status 6; tasks wider than the widest window keep
+ * status 1 (window) or 2 (pool).  work (host, optional, 8 values): executed
+ * band cells and snake compare steps, the aligner kernel time of each of the
+ * 4 window levels in microseconds (HIP events on the launch stream), and the
+ * number of task re-runs.  The library keeps its scratch between calls. */
+int mi_la_batch_device(const MiAlignParams *prm, const uint64_t *d_packed, long long npacked, const MiLaTask *d_tasks,
+                       int ntasks, const int *d_order, MiLaResult *d_results, uint16_t *d_traces, long long trace_cap,
+                       unsigned long long *d_trace_used, long long *work);
+
+/* Synthetic batched-aligner workload, generated in device memory (no
+ * reference counterpart: the input of the batched Local_Alignment benchmark).
+ * Read pairs: template length log-uniform in [lmin, lmax], pair error
+ * uniform in [emin_ppm, emax_ppm] parts per million split half into each
+ * read with insertion : deletion : substitution = 9 : 4 : 2, seed point on
+ * the true diagonal moved along B by up to +-offmax, COMP on half of them.
+ * Every decision is an integer hash of (seed, task, position), so the task
+ * set does not depend on the launch. */
+typedef struct {
+    unsigned long long seed;
+    int lmin, lmax;
+    int emin_ppm, emax_ppm;
+    int offmax;
+} MiGenTaskParams;
+
+/* Writes ntasks tasks to d_tasks and their 2-bit packed reads (A then B per
+ * task, laid out as MiLaTask describes) to d_packed (device pointers).
+ * *npacked receives the words needed; with d_packed == NULL only that is
+ * computed. */
+int mi_gen_la_tasks(const MiGenTaskParams *g, int ntasks, uint64_t *d_packed, long long packed_cap,
+                    long long *npacked, MiLaTask *d_tasks);
+
 /* One Compute_Trace_PTS task: the A-view path of an alignment (its uint16
  * trace points at `points` in the points array) between two 2-bit packed
  * reads laid out as for MiLaTask (the B read already complemented when the
@@ -196,6 +234,20 @@ typedef struct {
 int mi_trace_pts_host(int trace_spacing, int mode, const uint64_t *packed, long long npacked, const uint16_t *points,
                       long long npoints, const MiTraceTask *tasks, int ntasks, MiTraceResult *results, int *trace,
                       long long ntrace);
+
+/* Device memory for callers of the device-resident entry points (no
+ * reference counterpart): allocation (NULL on failure), release, blocking
+ * copies host -> device (to_device = 1) or device -> host (0), and fills. */
+void *mi_device_alloc(long long bytes);
+int mi_device_free(void *p);
+int mi_device_copy(void *dst, const void *src, long long bytes, int to_device);
+int mi_device_memset(void *p, int value, long long bytes);
+
+/* Raises (1) or lowers (0) the stop flag every long-running kernel of the
+ * library polls (no reference counterpart).  Kernels in flight and launched
+ * while it is raised stop at their next poll; their calls return
+ * MI_ERR_TIMEOUT because tasks were left unrun. */
+int mi_abort_device_work(int raise);
 
 /* Alignment parameters New_Align_Spec derives (ave_path, mscore, dscore). */
 int mi_align_params(const MiAlignParams *prm, int *ave_path, int *mscore, int *dscore);

@@ -87,3 +87,100 @@ def trace_pts(ts, paths, mode: int):
             raise RuntimeError(f"task {i}: trace status {r.status}")
         out[i] = (r.diffs, tr[r.trace:r.trace + r.ntrace].copy())
     return out
+
+
+# ---- device-resident batches (the batched-aligner workload) ---------------------
+
+TASK_INTS = C.sizeof(MiLaTask) // 4
+RES_INTS = C.sizeof(MiLaResult) // 4
+
+
+class DeviceTasks:
+    """Tasks and their packed reads in device memory, as mi_gen_la_tasks writes them."""
+
+    def __init__(self, packed, npacked, tasks, n, params=None):
+        self.packed = packed      # DeviceBuffer, npacked uint64 words
+        self.npacked = npacked
+        self.tasks = tasks        # DeviceBuffer, n MiLaTask records
+        self.n = n
+        self.params = params
+
+    def task_fields(self, count=None):
+        """Host copy of the first `count` task records as int32 [count, 14]
+        (aword and bword as two ints each, then alen, blen, comp, low, hgh,
+        anti, lbord, hbord, selfie, pad)."""
+        import numpy as np
+        n = self.n if count is None else count
+        return self.tasks.download(np.int32, n * TASK_INTS).reshape(n, TASK_INTS)
+
+    def words(self):
+        import numpy as np
+        return self.packed.download(np.uint64, self.npacked)
+
+
+def gen_tasks_device(n: int, seed: int, lmin: int = 1000, lmax: int = 60000, emin: float = 0.05,
+                     emax: float = 0.20, offmax: int = 200) -> DeviceTasks:
+    """n Local_Alignment tasks generated on the GPU (mi_gen_la_tasks)."""
+    from .lib import DeviceBuffer, MiGenTaskParams
+
+    g = MiGenTaskParams()
+    g.seed, g.lmin, g.lmax = seed, lmin, lmax
+    g.emin_ppm, g.emax_ppm, g.offmax = int(round(emin * 1e6)), int(round(emax * 1e6)), offmax
+    npk = C.c_longlong(0)
+    rc = lib.mi_gen_la_tasks(C.byref(g), n, None, 0, C.byref(npk), None)
+    if rc != 0:
+        raise RuntimeError(f"mi_gen_la_tasks failed ({rc}): {lib.last_error()}")
+    packed = DeviceBuffer(8 * npk.value)
+    tasks = DeviceBuffer(n * C.sizeof(MiLaTask))
+    rc = lib.mi_gen_la_tasks(C.byref(g), n, packed.ptr, npk.value, C.byref(npk), tasks.ptr)
+    if rc != 0:
+        raise RuntimeError(f"mi_gen_la_tasks failed ({rc}): {lib.last_error()}")
+    return DeviceTasks(packed, npk.value, tasks, n, g)
+
+
+def trace_bound(fields, tspace: int) -> int:
+    """Trace values a task set can produce: 4 (max(alen, blen) / tspace + 2) per task."""
+    import numpy as np
+    mx = np.maximum(fields[:, 4], fields[:, 5]).astype(np.int64)
+    return int((4 * (mx // tspace + 2)).sum()) + 64
+
+
+class DeviceBatchResult:
+    def __init__(self, results, n, arena, used, work):
+        self.results = results    # DeviceBuffer, n MiLaResult records
+        self.n = n
+        self.arena = arena        # DeviceBuffer of uint16 trace values
+        self.used = used          # trace values written
+        self.work = work          # executed cells, snake, kernel us per level (4), re-runs
+
+    def fields(self):
+        """Host copy of the results as a structured numpy array (MiLaResult fields)."""
+        import numpy as np
+        dt = np.dtype([(name, np.int32 if ct is C.c_int else np.int64) for name, ct in MiLaResult._fields_])
+        return self.results.download(np.uint8, self.n * C.sizeof(MiLaResult)).view(dt)
+
+    def traces(self):
+        import numpy as np
+        return self.arena.download(np.uint16, self.used)
+
+
+def la_batch_device(dt: DeviceTasks, ave_corr: float = 0.70, tspace: int = 100, freq=(0.25, 0.25, 0.25, 0.25),
+                    order=None, count=None) -> DeviceBatchResult:
+    """Runs the first `count` tasks (all by default) of a DeviceTasks on the
+    GPU (mi_la_batch_device); the trace arena is sized by trace_bound."""
+    import numpy as np
+    from .lib import DeviceBuffer
+
+    n = dt.n if count is None else count
+    bound = trace_bound(dt.task_fields(n), tspace)
+    res = DeviceBuffer(n * C.sizeof(MiLaResult))
+    arena = DeviceBuffer(2 * bound)
+    used = DeviceBuffer(8)
+    used.zero()
+    work = (C.c_longlong * 8)()
+    prm = params(ave_corr, tspace, freq)
+    rc = lib.mi_la_batch_device(C.byref(prm), dt.packed.ptr, dt.npacked, dt.tasks.ptr, n,
+                                None if order is None else order.ptr, res.ptr, arena.ptr, bound, used.ptr, work)
+    if rc != 0:
+        raise RuntimeError(f"mi_la_batch_device failed ({rc}): {lib.last_error()}")
+    return DeviceBatchResult(res, n, arena, int(used.download(np.int64, 1)[0]), list(work))

@@ -114,7 +114,10 @@ int mi_wait_watchdog(hipStream_t st, const char* what)
     for (;;) {
         hipError_t q = hipStreamQuery(st);
         if (q == hipSuccess) break;
-        if (q != hipErrorNotReady) return -1;
+        if (q != hipErrorNotReady) {
+            if (raised) __atomic_store_n(g_abort_host, 0, __ATOMIC_SEQ_CST);
+            return -1;
+        }
         double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (!raised && el > limit) {
             fprintf(stderr, "%s: watchdog deadline %.0f s passed, stopping the kernel\n", what, limit);
@@ -144,6 +147,49 @@ int mi_device_count(void)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+void* mi_device_alloc(long long bytes)
+{
+    void* p = nullptr;
+    if (bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess) {
+        mi::mi_set_error("mi_device_alloc: device allocation failed");
+        return nullptr;
+    }
+    return p;
+}
+
+int mi_device_free(void* p)
+{
+    if (p && hipFree(p) != hipSuccess) return MI_ERR_DEVICE;
+    return 0;
+}
+
+int mi_device_copy(void* dst, const void* src, long long bytes, int to_device)
+{
+    if (bytes <= 0) return 0;
+    if (hipMemcpy(dst, src, (size_t)bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost) != hipSuccess) {
+        mi::mi_set_error("mi_device_copy: copy failed");
+        return MI_ERR_DEVICE;
+    }
+    return 0;
+}
+
+int mi_device_memset(void* p, int value, long long bytes)
+{
+    if (bytes <= 0) return 0;
+    if (hipMemset(p, value, (size_t)bytes) != hipSuccess) {
+        mi::mi_set_error("mi_device_memset: failed");
+        return MI_ERR_DEVICE;
+    }
+    return 0;
+}
+
+int mi_abort_device_work(int raise)
+{
+    if (mi::mi_abort_flag_device() == nullptr) return MI_ERR_DEVICE;
+    __atomic_store_n(mi::g_abort_host, raise ? 1 : 0, __ATOMIC_SEQ_CST);
+    return 0;
 }
 
 int mi_align_params(const MiAlignParams* prm, int* ave_path, int* mscore, int* dscore)

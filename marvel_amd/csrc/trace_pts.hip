@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "la_wave.hpp"
@@ -272,6 +273,45 @@ __global__ __launch_bounds__(64) void k_trace_pts(const TrTask* __restrict__ tas
     }
 }
 
+constexpr int TR_UNRUN = 7;   // the launch stopped before reaching the task
+
+__global__ void k_tr_mark_unrun(const int* __restrict__ order, int n, TrOut* __restrict__ outs)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) outs[order[i]].status = TR_UNRUN;
+}
+
+// device buffers of mi_trace_pts_host, kept between calls (one set per device)
+struct TrBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool grow(size_t need)
+    {
+        if (need <= bytes) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        const size_t b = std::max(need, (size_t)1 << 16);
+        if (hipMalloc(&p, b) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = b;
+        return true;
+    }
+};
+struct TrWorkspace {
+    TrBuf tasks, order, packed, points, trace, outs, ctr, scratch;
+};
+static std::mutex g_tr_lock;
+static TrWorkspace& tr_workspace()
+{
+    static TrWorkspace ws[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    return ws[dev];
+}
+
 }  // namespace mi
 
 using namespace mi;
@@ -279,6 +319,11 @@ using namespace mi;
 extern "C" {
 
 // Batched Compute_Trace_PTS over host arrays (see include/marvl_align.h).
+// The per-lane row area of each task is known on the host from its trace
+// points (the path-wide maxima the rows are sized by), so tasks are split
+// into a level that fits 2^15 ints per lane and a level sized by the largest
+// remaining task, with grids capped so the scratch stays within a budget.
+// Device buffers are kept between calls (grown, never shrunk).
 int mi_trace_pts_host(int trace_spacing, int mode, const uint64_t* packed, long long npacked, const uint16_t* points,
                       long long npoints, const MiTraceTask* tasks, int ntasks, MiTraceResult* results, int* trace,
                       long long ntrace)
@@ -286,6 +331,7 @@ int mi_trace_pts_host(int trace_spacing, int mode, const uint64_t* packed, long 
     if (ntasks <= 0) return 0;
     if (trace_spacing <= 0 || mode < -1 || mode > 1) return MI_ERR_ARG;
     std::vector<TrTask> tk(ntasks);
+    std::vector<long long> need(ntasks);
     long long tro = 0;
     for (int i = 0; i < ntasks; i++) {
         const MiTraceTask& s = tasks[i];
@@ -298,8 +344,17 @@ int mi_trace_pts_host(int trace_spacing, int mode, const uint64_t* packed, long 
         const int nseg = s.tlen >= 2 ? s.tlen / 2 : 1;
         if (nseg > 1 && (s.abpos / trace_spacing + nseg - 1) * trace_spacing >= s.aepos) return MI_ERR_ARG;
         long long bsum = s.bbpos;
-        for (int q = 1; q < s.tlen - 2; q += 2) bsum += points[s.points + q];
+        int dmax = 0, nmax = 0;
+        for (int q = 0; q + 1 < s.tlen; q += 2) {
+            dmax = std::max(dmax, (int)points[s.points + q]);
+            nmax = std::max(nmax, (int)points[s.points + q + 1]);
+            if (q + 2 < s.tlen - 1) bsum += points[s.points + q + 1];
+        }
         if (bsum > s.bepos) return MI_ERR_ARG;
+        if (s.tlen <= 1) nmax = s.bepos - s.bbpos;
+        // the kernel's row area: (2 (dmax + 3) + 2) rows of (TS + nmax + 3) ints plus the staging
+        const long long s_row = trace_spacing + nmax + 3;
+        need[i] = (2LL * (dmax + 3) + 2) * s_row + (trace_spacing + nmax + 16);
         tk[i].aword = s.aword;
         tk[i].bword = s.bword;
         tk[i].pts = s.points;
@@ -312,85 +367,106 @@ int mi_trace_pts_host(int trace_spacing, int mode, const uint64_t* packed, long 
         tk[i].out = tro;
         tro += (long long)(s.aepos - s.abpos) + (s.bepos - s.bbpos) + 16;
     }
-    const int cus = mi_num_cus();
-    struct Lv {
-        long long lane_cap;
-        int waves;
-    };
-    const Lv levels[2] = {{1LL << 15, cus * 4}, {1LL << 22, 16}};
-    TrTask* dtk = nullptr;
-    int* dorder = nullptr;
-    uint64_t* dpk = nullptr;
-    uint16_t* dpts = nullptr;
-    int* dtr = nullptr;
-    TrOut* dout = nullptr;
-    int* dscr = nullptr;
-    int* dctr = nullptr;
-    int rc = MI_ERR_DEVICE;
-    std::vector<TrOut> ho(ntasks);
-    std::vector<int> todo(ntasks);
-    for (int i = 0; i < ntasks; i++) todo[i] = i;
+    // two levels: tasks whose rows fit 2^15 ints per lane, then the rest
+    const long long kSmall = 1LL << 15;
+    const long long kBudget = 4LL << 30;   // scratch bytes per level
+    std::vector<int> lv[2];
+    long long cap[2] = {0, 0};
+    for (int i = 0; i < ntasks; i++) {
+        const int l = need[i] <= kSmall ? 0 : 1;
+        lv[l].push_back(i);
+        cap[l] = std::max(cap[l], need[i]);
+    }
+    std::lock_guard<std::mutex> g(g_tr_lock);
+    TrWorkspace& W = tr_workspace();
+    long long scr_ints = 0;
+    int grids[2] = {0, 0};
+    for (int l = 0; l < 2; l++) {
+        if (lv[l].empty()) continue;
+        const long long per_wave = cap[l] * 64;
+        long long gmax = std::max(1LL, kBudget / (per_wave * (long long)sizeof(int)));
+        grids[l] = (int)std::min<long long>(std::min<long long>(mi_num_cus() * 4, (long long)lv[l].size()), gmax);
+        scr_ints = std::max(scr_ints, per_wave * grids[l]);
+    }
+    if (!W.tasks.grow(sizeof(TrTask) * ntasks) || !W.order.grow(sizeof(int) * ntasks) ||
+        !W.packed.grow(sizeof(uint64_t) * npacked) || !W.points.grow(sizeof(uint16_t) * std::max(npoints, 1LL)) ||
+        !W.trace.grow(sizeof(int) * std::max(tro, 1LL)) || !W.outs.grow(sizeof(TrOut) * ntasks) ||
+        !W.ctr.grow(256) || !W.scratch.grow(sizeof(int) * (size_t)scr_ints)) {
+        mi_set_error("mi_trace_pts_host: device memory");
+        return MI_ERR_DEVICE;
+    }
+    TrTask* dtk = (TrTask*)W.tasks.p;
+    int* dorder = (int*)W.order.p;
+    TrOut* dout = (TrOut*)W.outs.p;
+    int* dtr = (int*)W.trace.p;
+    int* dctr = (int*)W.ctr.p;
     hipStream_t st = 0;
-    if (hipMalloc(&dtk, sizeof(TrTask) * ntasks) != hipSuccess) goto done;
-    if (hipMalloc(&dorder, sizeof(int) * ntasks) != hipSuccess) goto done;
-    if (hipMalloc(&dpk, sizeof(uint64_t) * npacked) != hipSuccess) goto done;
-    if (hipMalloc(&dpts, sizeof(uint16_t) * std::max(npoints, 1LL)) != hipSuccess) goto done;
-    if (hipMalloc(&dtr, sizeof(int) * std::max(tro, 1LL)) != hipSuccess) goto done;
-    if (hipMalloc(&dout, sizeof(TrOut) * ntasks) != hipSuccess) goto done;
-    if (hipMalloc(&dctr, sizeof(int)) != hipSuccess) goto done;
-    if (hipMemcpy(dtk, tk.data(), sizeof(TrTask) * ntasks, hipMemcpyHostToDevice) != hipSuccess) goto done;
-    if (hipMemcpy(dpk, packed, sizeof(uint64_t) * npacked, hipMemcpyHostToDevice) != hipSuccess) goto done;
-    if (npoints > 0 && hipMemcpy(dpts, points, sizeof(uint16_t) * npoints, hipMemcpyHostToDevice) != hipSuccess) goto done;
-    for (int level = 0; level < 2 && !todo.empty(); level++) {
-        const Lv& L = levels[level];
-        const int grid = (int)std::min<long long>(L.waves, (long long)todo.size());
-        if (dscr) (void)hipFree(dscr);
-        dscr = nullptr;
-        if (hipMalloc(&dscr, sizeof(int) * (size_t)L.lane_cap * 64 * grid) != hipSuccess) goto done;
-        if (hipMemcpy(dorder, todo.data(), sizeof(int) * todo.size(), hipMemcpyHostToDevice) != hipSuccess) goto done;
-        if (hipMemset(dctr, 0, sizeof(int)) != hipSuccess) goto done;
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_pts), dim3(grid), dim3(64), 0, st, dtk, dorder, (int)todo.size(),
-                           dpk, dpts, trace_spacing, mode, dtr, dout, dscr, L.lane_cap, dctr, mi_abort_flag_device());
-        if (hipGetLastError() != hipSuccess) goto done;
-        {
-            const int wd = mi_wait_watchdog(st, "mi_trace_pts_host");
-            if (wd > 0) { rc = MI_ERR_TIMEOUT; goto done; }
-            if (wd < 0) goto done;
-        }
-        if (hipMemcpy(ho.data(), dout, sizeof(TrOut) * ntasks, hipMemcpyDeviceToHost) != hipSuccess) goto done;
-        std::vector<int> again;
-        for (int i : todo)
-            if (ho[i].status == TR_BIG) again.push_back(i);
-        todo.swap(again);
+    const int* flag = mi_abort_flag_device();
+    if (flag == nullptr) return MI_ERR_DEVICE;
+    if (hipMemcpy(dtk, tk.data(), sizeof(TrTask) * ntasks, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(W.packed.p, packed, sizeof(uint64_t) * npacked, hipMemcpyHostToDevice) != hipSuccess ||
+        (npoints > 0 &&
+         hipMemcpy(W.points.p, points, sizeof(uint16_t) * npoints, hipMemcpyHostToDevice) != hipSuccess)) {
+        mi_set_error("mi_trace_pts_host: HIP error");
+        return MI_ERR_DEVICE;
     }
-    {
-        std::vector<int> htr(std::max(tro, 1LL));
-        if (hipMemcpy(htr.data(), dtr, sizeof(int) * std::max(tro, 1LL), hipMemcpyDeviceToHost) != hipSuccess) goto done;
-        long long o = 0;
-        for (int i = 0; i < ntasks; i++) {
-            MiTraceResult& r = results[i];
-            r.status = ho[i].status;
-            r.diffs = ho[i].diffs;
-            r.ntrace = ho[i].status == TR_OK ? ho[i].ntrace : 0;
-            r.trace = o;
-            if (o + r.ntrace > ntrace) { rc = MI_ERR_ARG; goto done; }
-            memcpy(trace + o, htr.data() + tk[i].out, sizeof(int) * r.ntrace);
-            o += r.ntrace;
+    std::vector<TrOut> ho(ntasks);
+    for (int l = 0; l < 2; l++) {
+        const int n = (int)lv[l].size();
+        if (n == 0) continue;
+        if (hipMemcpy(dorder, lv[l].data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(dctr, 0, sizeof(int)) != hipSuccess) {
+            mi_set_error("mi_trace_pts_host: HIP error");
+            return MI_ERR_DEVICE;
+        }
+        hipLaunchKernelGGL(k_tr_mark_unrun, dim3((n + 255) / 256), dim3(256), 0, st, dorder, n, dout);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_pts), dim3(grids[l]), dim3(64), 0, st, dtk, dorder, n,
+                           (const uint64_t*)W.packed.p, (const uint16_t*)W.points.p, trace_spacing, mode, dtr, dout,
+                           (int*)W.scratch.p, cap[l], dctr, flag);
+        if (hipGetLastError() != hipSuccess) {
+            mi_set_error("mi_trace_pts_host: launch failed");
+            return MI_ERR_DEVICE;
+        }
+        const int wd = mi_wait_watchdog(st, "mi_trace_pts_host");
+        if (wd > 0) {
+            mi_set_error("mi_trace_pts_host: kernel exceeded the watchdog deadline and was stopped");
+            return MI_ERR_TIMEOUT;
+        }
+        if (wd < 0) {
+            mi_set_error("mi_trace_pts_host: HIP error");
+            return MI_ERR_DEVICE;
         }
     }
-    rc = 0;
-done:
-    if (rc == MI_ERR_DEVICE) mi_set_error("mi_trace_pts_host: HIP error");
-    if (rc == MI_ERR_TIMEOUT) mi_set_error("mi_trace_pts_host: kernel exceeded the watchdog deadline and was stopped");
-    (void)hipFree(dtk);
-    (void)hipFree(dorder);
-    (void)hipFree(dpk);
-    (void)hipFree(dpts);
-    (void)hipFree(dtr);
-    (void)hipFree(dout);
-    (void)hipFree(dscr);
-    (void)hipFree(dctr);
-    return rc;
+    std::vector<int> htr(std::max(tro, 1LL));
+    if (hipMemcpy(ho.data(), dout, sizeof(TrOut) * ntasks, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(htr.data(), dtr, sizeof(int) * std::max(tro, 1LL), hipMemcpyDeviceToHost) != hipSuccess) {
+        mi_set_error("mi_trace_pts_host: HIP error");
+        return MI_ERR_DEVICE;
+    }
+    long long o = 0;
+    for (int i = 0; i < ntasks; i++) {
+        const TrOut& h = ho[i];
+        if (h.status == TR_UNRUN || h.status == TR_BIG) {
+            mi_set_error(h.status == TR_UNRUN ? "mi_trace_pts_host: a launch stopped before reaching every task"
+                                              : "mi_trace_pts_host: a task's rows exceed its level's area");
+            return MI_ERR_TIMEOUT;
+        }
+        MiTraceResult& r = results[i];
+        r.status = h.status;
+        r.diffs = h.diffs;
+        r.ntrace = h.status == TR_OK ? h.ntrace : 0;
+        r.trace = o;
+        // a path's trace holds at most one entry per base of its A and B spans
+        const long long slice = (long long)(tk[i].aepos - tk[i].abpos) + (tk[i].bepos - tk[i].bbpos) + 16;
+        if (r.ntrace < 0 || r.ntrace > slice) {
+            mi_set_error("mi_trace_pts_host: trace longer than its slice");
+            return MI_ERR_DEVICE;
+        }
+        if (o + r.ntrace > ntrace) return MI_ERR_ARG;
+        memcpy(trace + o, htr.data() + tk[i].out, sizeof(int) * r.ntrace);
+        o += r.ntrace;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -31,6 +31,11 @@ class MiLaResult(C.Structure):
                 ("snake", C.c_longlong)]
 
 
+class MiGenTaskParams(C.Structure):
+    _fields_ = [("seed", C.c_ulonglong), ("lmin", C.c_int), ("lmax", C.c_int), ("emin_ppm", C.c_int),
+                ("emax_ppm", C.c_int), ("offmax", C.c_int)]
+
+
 class MiTraceTask(C.Structure):
     _fields_ = [("aword", C.c_longlong), ("bword", C.c_longlong), ("alen", C.c_int), ("blen", C.c_int),
                 ("abpos", C.c_int), ("bbpos", C.c_int), ("aepos", C.c_int), ("bepos", C.c_int),
@@ -47,7 +52,8 @@ EXPORTS = [
     "New_Work_Data", "Free_Work_Data", "Local_Alignment",
     "Read_Overlap", "Read_Trace", "Write_Overlap", "Compress_TraceTo8", "Decompress_TraceTo16",
     "Check_Trace_Points",
-    "mi_la_batch_host", "mi_align_params", "mi_last_error", "mi_device_count", "mi_selftest",
+    "mi_la_batch_host", "mi_la_batch_device", "mi_gen_la_tasks", "mi_abort_device_work", "mi_device_alloc", "mi_device_free",
+    "mi_device_copy", "mi_device_memset", "mi_align_params", "mi_last_error", "mi_device_count", "mi_selftest",
 ]
 
 
@@ -64,6 +70,18 @@ class _Lib:
                                            C.POINTER(MiLaTask), C.c_int, C.POINTER(MiLaResult),
                                            C.c_void_p, C.c_longlong]
             h.mi_la_batch_host.restype = C.c_int
+            h.mi_la_batch_device.argtypes = [C.POINTER(MiAlignParams), C.c_void_p, C.c_longlong, C.c_void_p, C.c_int,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p,
+                                             C.POINTER(C.c_longlong)]
+            h.mi_la_batch_device.restype = C.c_int
+            h.mi_gen_la_tasks.argtypes = [C.POINTER(MiGenTaskParams), C.c_int, C.c_void_p, C.c_longlong,
+                                          C.POINTER(C.c_longlong), C.c_void_p]
+            h.mi_gen_la_tasks.restype = C.c_int
+            h.mi_device_alloc.argtypes = [C.c_longlong]
+            h.mi_device_alloc.restype = C.c_void_p
+            h.mi_device_free.argtypes = [C.c_void_p]
+            h.mi_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int]
+            h.mi_device_memset.argtypes = [C.c_void_p, C.c_int, C.c_longlong]
             h.mi_align_params.argtypes = [C.POINTER(MiAlignParams), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]
             h.mi_last_error.restype = C.c_char_p
@@ -95,3 +113,43 @@ def params(ave_corr: float, trace_space: int, freq) -> MiAlignParams:
     for i in range(4):
         p.freq[i] = float(freq[i])
     return p
+
+
+class DeviceBuffer:
+    """A device allocation of the library (mi_device_alloc); freed with the object."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib.mi_device_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise MemoryError(f"device allocation of {self.nbytes} bytes: {lib.last_error()}")
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p:
+            lib.mi_device_free(p)
+            self.ptr = None
+
+    def offset(self, nbytes: int) -> int:
+        return self.ptr + int(nbytes)
+
+    def upload(self, arr, at: int = 0):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        if at + a.nbytes > self.nbytes:
+            raise ValueError("upload past the end of the buffer")
+        if lib.mi_device_copy(self.ptr + at, a.ctypes.data, a.nbytes, 1) != 0:
+            raise RuntimeError(lib.last_error())
+
+    def download(self, dtype, count: int, at: int = 0):
+        import numpy as np
+        out = np.empty(int(count), dtype=dtype)
+        if at + out.nbytes > self.nbytes:
+            raise ValueError("download past the end of the buffer")
+        if out.nbytes and lib.mi_device_copy(out.ctypes.data, self.ptr + at, out.nbytes, 0) != 0:
+            raise RuntimeError(lib.last_error())
+        return out
+
+    def zero(self):
+        if lib.mi_device_memset(self.ptr, 0, self.nbytes) != 0:
+            raise RuntimeError(lib.last_error())

@@ -110,3 +110,24 @@ def trace_segment(a: np.ndarray, b: np.ndarray, tspace: int, mode: int):
     if rc:
         raise RuntimeError(f"or_trace_pts {rc}")
     return df.value, out[:nt.value].copy()
+
+
+def gen_la_pair(seed, task, lmin=1000, lmax=60000, emin_ppm=50000, emax_ppm=200000, offmax=200):
+    """The batched-aligner workload generator restated on the CPU: (A bases,
+    B bases, par[8], stats[10]) of one task (oracle/la_gen_oracle.c)."""
+    h = handle()
+    if not getattr(h, "_gen_ready", False):
+        h.or_gen_la_pair.restype = C.c_int
+        h.or_gen_la_pair.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        h._gen_ready = True
+    cap = 2 * lmax + 16
+    a = np.zeros(cap, dtype=np.uint8)
+    b = np.zeros(cap, dtype=np.uint8)
+    par = np.zeros(8, dtype=np.int32)
+    st = np.zeros(10, dtype=np.int32)
+    rc = h.or_gen_la_pair(seed, task, lmin, lmax, emin_ppm, emax_ppm, offmax, a.ctypes.data, cap, b.ctypes.data, cap,
+                          par.ctypes.data, st.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("or_gen_la_pair capacity")
+    return a[:par[0]].copy(), b[:par[1]].copy(), par, st

@@ -186,6 +186,19 @@ int mi_la_batch_device(const MiAlignParams *prm, const uint64_t *d_packed, long 
                        int ntasks, const int *d_order, MiLaResult *d_results, uint16_t *d_traces, long long trace_cap,
                        unsigned long long *d_trace_used, long long *work);
 
+/* mi_la_batch_device without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  All scratch -- the
+ * alignment tables, counters, retry lists and per-wave pebble pools -- lives
+ * in one file-scope device array of 64 MiB per GPU, held until the process
+ * ends; the caller passes inputs and outputs as for mi_la_batch_device (task
+ * order only).  At most 1048576 tasks per call (more: MI_ERR_ARG); fewer
+ * waves run than in mi_la_batch_device, so it is slower.  Every call on one
+ * GPU shares the array: calls that overlap on one GPU race. */
+int mi_la_batch_device_noalloc(const MiAlignParams *prm, const uint64_t *d_packed, long long npacked,
+                               const MiLaTask *d_tasks, int ntasks, MiLaResult *d_results, uint16_t *d_traces,
+                               long long trace_cap, unsigned long long *d_trace_used);
+
 /* Synthetic batched-aligner workload, generated in device memory (no
  * reference counterpart: the input of the batched Local_Alignment benchmark).
  * Read pairs: template length log-uniform in [lmin, lmax], pair error

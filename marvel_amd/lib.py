@@ -52,7 +52,7 @@ EXPORTS = [
     "New_Work_Data", "Free_Work_Data", "Local_Alignment",
     "Read_Overlap", "Read_Trace", "Write_Overlap", "Compress_TraceTo8", "Decompress_TraceTo16",
     "Check_Trace_Points",
-    "mi_la_batch_host", "mi_la_batch_device", "mi_gen_la_tasks", "mi_abort_device_work", "mi_device_alloc", "mi_device_free",
+    "mi_la_batch_host", "mi_la_batch_device", "mi_la_batch_device_noalloc", "mi_gen_la_tasks", "mi_abort_device_work", "mi_device_alloc", "mi_device_free",
     "mi_device_copy", "mi_device_memset", "mi_align_params", "mi_last_error", "mi_device_count", "mi_selftest",
 ]
 
@@ -74,6 +74,9 @@ class _Lib:
                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p,
                                              C.POINTER(C.c_longlong)]
             h.mi_la_batch_device.restype = C.c_int
+            h.mi_la_batch_device_noalloc.argtypes = [C.POINTER(MiAlignParams), C.c_void_p, C.c_longlong, C.c_void_p,
+                                                     C.c_int, C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
+            h.mi_la_batch_device_noalloc.restype = C.c_int
             h.mi_gen_la_tasks.argtypes = [C.POINTER(MiGenTaskParams), C.c_int, C.c_void_p, C.c_longlong,
                                           C.POINTER(C.c_longlong), C.c_void_p]
             h.mi_gen_la_tasks.restype = C.c_int

@@ -95,6 +95,16 @@ void Write_Overlap(FILE *output, Overlap *ovl, int tbytes);
 void Compress_TraceTo8(Overlap *ovl);
 void Decompress_TraceTo16(Overlap *ovl);
 int Check_Trace_Points(Overlap *ovl, int tspace, int verbose, char *fname);
+/* Print_Overlap: replaces dalign/align.c:3372-3417 (dalign/align.h:313); same text. */
+void Print_Overlap(FILE *output, Overlap *ovl, int tbytes, int indent);
+
+/* Complement_Seq: replaces dalign/align.c:3530-3546 (dalign/align.h:119): reverse
+ * complement of a read of bases 0..3, in place.
+ * Flip_Alignment: replaces dalign/align.c:3451-3522 (dalign/align.h:230-234): the
+ * alignment with A and B exchanged; full = 1 also converts an int trace (as
+ * Compute_Trace_PTS leaves it). */
+void Complement_Seq(char *a, int n);
+void Flip_Alignment(Alignment *align, int full);
 
 /* Per-thread overlap buffers of an Align_Spec (dalign/align.h:324-336). */
 typedef struct {
@@ -255,6 +265,24 @@ void *mi_device_alloc(long long bytes);
 int mi_device_free(void *p);
 int mi_device_copy(void *dst, const void *src, long long bytes, int to_device);
 int mi_device_memset(void *p, int value, long long bytes);
+
+/* Launch shapes of the GPU kernels (no reference counterpart; results do
+ * not depend on them): waves per CU of the report kernel's 64-diagonal
+ * window (4..32, default 32) and of its 128-diagonal window (2..32, default
+ * 16), the pebble pool of the 64-diagonal window as a power of two (12..18,
+ * default 15), waves per CU of the batched aligner (4..32, default 32).
+ * A 0 field keeps the default; NULL restores all defaults.  The tuner in
+ * marvel_amd/tune.py picks them from a calibration run. */
+typedef struct {
+    int report_waves_per_cu;
+    int report_pool_log2;
+    int report_l1_waves_per_cu;
+    int batch_waves_per_cu;
+} MiTuning;
+int mi_set_tuning(const MiTuning *t);
+int mi_get_tuning(MiTuning *t);
+/* The current device's architecture name (e.g. gfx950) and compute units. */
+int mi_device_info(char *name, int cap, int *cus);
 
 /* Raises (1) or lowers (0) the stop flag every long-running kernel of the
  * library polls (no reference counterpart).  Kernels in flight and launched

@@ -124,8 +124,18 @@ typedef struct {
     long long seed_sort_passes;     /* 8-bit digit passes of that sort */
     long long njobs_kept;           /* pairs past the exact gate pre-check (the report kernel's jobs) */
     long long wave_steps;           /* aligner wave steps (one per difference level and pass) */
-    long long cand_scored;          /* candidates the scorer evaluated (gating mode) */
-    long long cand_skipped;         /* candidates it kept from being aligned */
+    long long cand_scored;          /* read pairs the scorer evaluated (gating mode) */
+    long long cand_skipped;         /* read pairs it dropped before alignment */
+    /* with MI_REDUND_STATS=1 in the environment: how often Handle_Redundancies
+     * (dalign/filter.c:1403-1566) took each branch, final runs only --
+     * 0-2: alignment j starts first and k reaches further, fused (A and B
+     * views with COMP / without / A view only); 4: j contains k; 5: same
+     * start, j longer; 6-8: k starts first and j reaches further, fused (as
+     * 0-2); 3 unused */
+    long long redund[9];
+    /* radix sorts and scans recorded as HIP graphs by this pass, replayed
+     * from earlier passes, and the kernel launches those replays stood for */
+    long long graphs_captured, graphs_replayed, launches_replayed;
 } MiFilterStats;
 
 /* Match_Filter returning an error code (MI_ERR_*) instead of exiting. */
@@ -135,21 +145,38 @@ void mi_filter_stats(MiFilterStats *out);
 
 /* ---- candidate scorer (no reference counterpart) ---------------------------
  * An MLP 16 -> 64 -> 64 -> 1 (ReLU; bf16 weights and activations, fp32
- * accumulation, gfx950 MFMA) over 16 features of each alignment candidate of
- * report_thread (a seed that passes the band gate, dalign/filter.c:1772-1778).
+ * accumulation, gfx950 MFMA) over 16 features of each alignment candidate:
+ * a read pair that passes the exact gate pre-check of report_thread's band
+ * gate (dalign/filter.c:1740-1778), described by its seed count and its
+ * strongest diagonal bins (seeds, apos span, mean / sd of the seed spacing,
+ * read lengths, the overlap the bin's diagonal implies, short-period k-mers).
  * Weights are input-major: w1[16*64], w2[64*64], w3[64].  Mode 0 aligns every
- * candidate (bit-exact with the reference), 1 does the same and records each
- * aligned candidate's features with label 1 when its alignment was kept
- * ((aepos-abpos)+(bepos-bbpos) >= 2*minlen), 2 skips candidates whose logit
- * is below the threshold. */
+ * pair (bit-exact with the reference); 1 does the same and records each
+ * pair's features, the records its exact run wrote and the aligner work it
+ * took; 2 drops the pairs whose logit is below the threshold before
+ * alignment (every kept pair runs the exact report kernel). */
 int mi_scorer_load(const float *w1, const float *b1, const float *w2, const float *b2, const float *w3, float b3,
                    float threshold);
 int mi_scorer_mode(int mode);
 long long mi_scorer_captured(void);
-/* copies up to cap captured rows (16 floats + a label each) and clears the store */
-long long mi_scorer_take(float *feat, int *label, long long cap);
+/* copies up to cap captured rows (16 floats, the record count and the work,
+ * band cells + snake steps, of each) and clears the store */
+long long mi_scorer_take(float *feat, int *nrec, unsigned long long *work, long long cap);
 /* logits of n feature rows with the loaded weights, on the GPU */
 int mi_mlp_score(const float *feat, long long n, float *logits);
+
+/* Training on the GPU (bf16 MFMA forward and backward, fp32 Adam).  The
+ * parameter vector holds 5313 floats: w1[16*64], b1[64], w2[64*64], b2[64],
+ * w3[64], b3 (input-major).  Loss: binary cross-entropy on the logits with
+ * per-row weights, averaged by the weights.
+ * mi_scorer_grad: gradient of that loss over all n rows at `params` (and the
+ * loss); mi_scorer_train: `epochs` passes of Adam (lr; beta1 0.9, beta2
+ * 0.999) over shuffled minibatches of `batch` rows, params updated in place,
+ * the weighted mean loss of each epoch in epoch_loss (optional). */
+int mi_scorer_grad(const float *feat, const float *label, const float *weight, long long n, const float *params,
+                   float *grad, float *loss);
+int mi_scorer_train(const float *feat, const float *label, const float *weight, long long n, int epochs, int batch,
+                    float lr, unsigned long long seed, float *params, float *epoch_loss);
 
 /* Index of the reverse complement of the block indexed by `table`, built on
  * the GPU from the packed reads the handle holds (the complement the

@@ -16,7 +16,7 @@ ARCH = os.environ.get("MI_OFFLOAD_ARCH", "gfx950")
 DEV_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
-LIB_SOURCES = ["la_batch.hip", "la_batch_noalloc.hip", "la_gen.hip", "filter.hip", "trace_pts.hip", "scorer.hip", "dust.hip", "selftest.hip", "spec.cpp", "align_capi.cpp", "ovl_buffer.cpp",
+LIB_SOURCES = ["la_batch.hip", "la_batch_noalloc.hip", "la_gen.hip", "filter.hip", "trace_pts.hip", "scorer.hip", "scorer_train.hip", "dust.hip", "selftest.hip", "spec.cpp", "align_capi.cpp", "ovl_buffer.cpp",
                "db_block.cpp", "tracks.cpp"]
 CLI_SOURCE = "daligner_main.cpp"
 

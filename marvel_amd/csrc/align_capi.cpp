@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/marvl_align.h"
@@ -235,6 +236,83 @@ int Check_Trace_Points(Overlap* ovl, int tspace, int verbose, char* fname)
         return 1;
     }
     return 0;
+}
+
+}  // extern "C"
+
+// Print_Overlap (dalign/align.h:313): the record's reads, intervals and
+// differences, then its trace as "diffs/B position" pairs, the B position
+// running from bbpos, a line break before every pair whose value index is
+// a multiple of 10.
+template <typename T>
+static void print_trace(FILE* out, const Overlap* ovl, const T* t, int indent)
+{
+    int p = ovl->path.bbpos + t[1];
+    fprintf(out, "%*sTrace: %3d/%5d", indent, "", (int)t[0], p);
+    for (int i = 3; i < ovl->path.tlen; i += 2) {
+        if (i % 10 == 0) fprintf(out, "\n%*s", indent + 6, "");
+        p += t[i];
+        fprintf(out, " %3d/%5d", (int)t[i - 1], p);
+    }
+    fprintf(out, "\n");
+}
+
+extern "C" {
+
+void Print_Overlap(FILE* output, Overlap* ovl, int tbytes, int indent)
+{
+    fprintf(output, "%*s%d vs. ", indent, "", ovl->aread);
+    if (COMP(ovl->flags)) fprintf(output, "c(%d)\n", ovl->bread);
+    else fprintf(output, "%d\n", ovl->bread);
+    fprintf(output, "%*s  [%d,%d] vs [%d,%d] w. %d diffs\n", indent, "", ovl->path.abpos, ovl->path.aepos,
+            ovl->path.bbpos, ovl->path.bepos, ovl->path.diffs);
+    if (ovl->path.trace == NULL) return;
+    if (tbytes == 1) print_trace(output, ovl, (const uint8_t*)ovl->path.trace, indent);
+    else print_trace(output, ovl, (const uint16_t*)ovl->path.trace, indent);
+}
+
+// Complement_Seq (dalign/align.h:119): reverse complement of bases 0..3 in
+// place (an involution).
+void Complement_Seq(char* a, int n)
+{
+    for (int i = 0, j = n - 1; i <= j; i++, j--) {
+        const char x = a[i], y = a[j];
+        a[i] = (char)(3 - y);
+        a[j] = (char)(3 - x);
+    }
+}
+
+// Flip_Alignment (dalign/align.h:230-234): the same alignment with the roles
+// of A and B exchanged.  Without COMP the intervals swap and (full) every
+// indel position of an int trace changes sign.  With COMP the intervals are
+// mirrored into the other read's complemented coordinates and (full) an
+// indel position p < 0 (a gap in A at B position -p) becomes alen + 2 + p,
+// a position p > 0 becomes p - (blen + 2), the trace read backwards.
+void Flip_Alignment(Alignment* align, int full)
+{
+    Path* path = align->path;
+    const int alen = align->alen, blen = align->blen;
+    int* trace = (int*)path->trace;
+    const int tlen = path->tlen;
+    if (COMP(align->flags)) {
+        const int ab = path->abpos, ae = path->aepos, bb = path->bbpos, be = path->bepos;
+        path->abpos = blen - be;
+        path->aepos = blen - bb;
+        path->bbpos = alen - ae;
+        path->bepos = alen - ab;
+        if (full) {
+            for (int i = 0; i < tlen; i++) trace[i] = trace[i] < 0 ? alen + 2 + trace[i] : trace[i] - (blen + 2);
+            for (int i = 0, j = tlen - 1; i < j; i++, j--) std::swap(trace[i], trace[j]);
+        }
+    } else {
+        std::swap(path->abpos, path->bbpos);
+        std::swap(path->aepos, path->bepos);
+        if (full)
+            for (int i = 0; i < tlen; i++) trace[i] = -trace[i];
+    }
+    std::swap(align->aseq, align->bseq);
+    align->alen = blen;
+    align->blen = alen;
 }
 
 }  // extern "C"

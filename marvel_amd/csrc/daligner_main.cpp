@@ -80,6 +80,56 @@ static void print_stats(const char* what, double secs)
     fflush(stdout);
 }
 
+// The learned scorer's weights file (marvel_amd/data/scorer_weights.json
+// layout: arrays "w1", "b1", "w2", "b2", "w3" and numbers "b3",
+// "threshold"): the values of one key.
+static bool json_floats(const std::string& js, const char* key, std::vector<float>& out)
+{
+    const std::string k = std::string("\"") + key + "\":";
+    size_t p = js.find(k);
+    if (p == std::string::npos) return false;
+    p += k.size();
+    while (p < js.size() && js[p] == ' ') p++;
+    out.clear();
+    const bool arr = p < js.size() && js[p] == '[';
+    if (arr) p++;
+    const char* s = js.c_str() + p;
+    for (;;) {
+        while (*s == ' ' || *s == ',') s++;
+        if (*s == ']' || *s == 0) break;
+        char* e = nullptr;
+        const double v = strtod(s, &e);
+        if (e == s) return false;
+        out.push_back((float)v);
+        s = e;
+        if (!arr) break;
+    }
+    return !out.empty();
+}
+
+// MI_SCORER_WEIGHTS=<file>: gate candidate pairs with the learned scorer
+// (not bit-exact with the reference; the default is the exact path)
+static bool load_scorer_from_env()
+{
+    const char* f = getenv("MI_SCORER_WEIGHTS");
+    if (f == nullptr || *f == 0) return true;
+    FILE* fp = fopen(f, "rb");
+    if (fp == nullptr) return false;
+    std::string js;
+    char buf[65536];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, fp)) > 0) js.append(buf, n);
+    fclose(fp);
+    std::vector<float> w1, b1, w2, b2, w3, b3, thr;
+    if (!json_floats(js, "w1", w1) || !json_floats(js, "b1", b1) || !json_floats(js, "w2", w2) ||
+        !json_floats(js, "b2", b2) || !json_floats(js, "w3", w3) || !json_floats(js, "b3", b3) ||
+        !json_floats(js, "threshold", thr) || w1.size() != 16 * 64 || b1.size() != 64 || w2.size() != 64 * 64 ||
+        b2.size() != 64 || w3.size() != 64)
+        return false;
+    return mi_scorer_load(w1.data(), b1.data(), w2.data(), b2.data(), w3.data(), b3[0], thr[0]) == 0 &&
+           mi_scorer_mode(2) == 0;
+}
+
 int main(int argc, char** argv)
 {
     int kmer = 14, binshift = 6, hitmin = 35, suppress = 0, tspace = 100, run_id = 1, no_trace = 0, l = 1000;
@@ -159,6 +209,10 @@ int main(int argc, char** argv)
     }
     if (Set_Filter_Params(kmer, binshift, suppress, hitmin)) {
         fprintf(stderr, "daligner: invalid k-mer length %d (%s)\n", kmer, mi_last_error());
+        return 1;
+    }
+    if (!load_scorer_from_env()) {
+        fprintf(stderr, "daligner: cannot load the scorer weights of MI_SCORER_WEIGHTS (%s)\n", mi_last_error());
         return 1;
     }
     const char* afile = argv[optind];

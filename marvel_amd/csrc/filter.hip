@@ -36,6 +36,7 @@
 #include "mi_common.h"
 #include "radix.hpp"
 #include "report.hpp"
+#include "scorer.hpp"
 
 extern "C" {
 int BIASED = 0;
@@ -190,6 +191,79 @@ static T d2h1(const T* dp)
     return v;
 }
 
+// ---- hipGraph replay of fixed launch sequences ----------------------------------------
+// A radix sort or a scan is a fixed sequence of launches on the filter stream
+// with no host synchronisation and no memset, determined by its arguments and
+// the addresses of its buffers.  The first call with a given key records the
+// sequence by stream capture and instantiates it; a later call with the same
+// key (the same block pair compared again, or the next B block of the same
+// size through the caching allocator's reused buffers) replays the
+// instantiated graph with one launch.  MI_GRAPHS=0 turns replay off.
+
+struct GraphKey {
+    uint64_t w[12];
+    bool operator<(const GraphKey& o) const { return memcmp(w, o.w, sizeof w) < 0; }
+};
+
+struct GraphCache {
+    std::map<GraphKey, hipGraphExec_t> execs;
+    std::vector<GraphKey> order;   // insertion order, for eviction
+    long long captured = 0, replayed = 0, launches_replayed = 0;
+    static constexpr size_t kMax = 96;
+    bool enabled()
+    {
+        static int on = -1;
+        if (on < 0) {
+            const char* e = getenv("MI_GRAPHS");
+            on = (e && atoi(e) == 0) ? 0 : 1;
+        }
+        return on == 1;
+    }
+    // runs `seq` (which enqueues `nlaunch` kernels on st) through the cache
+    template <class F>
+    void run(const GraphKey& k, hipStream_t st, int nlaunch, F seq)
+    {
+        if (!enabled()) {
+            seq();
+            return;
+        }
+        auto it = execs.find(k);
+        if (it != execs.end()) {
+            HC(hipGraphLaunch(it->second, st));
+            replayed += 1;
+            launches_replayed += nlaunch;
+            return;
+        }
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            seq();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HC(hipStreamEndCapture(st, &g));
+        HC(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        HC(hipGraphLaunch(ex, st));
+        captured += 1;
+        if (execs.size() >= kMax) {   // drop the oldest
+            auto old = execs.find(order.front());
+            if (old != execs.end()) {
+                HC(hipStreamSynchronize(st));
+                (void)hipGraphExecDestroy(old->second);
+                execs.erase(old);
+            }
+            order.erase(order.begin());
+        }
+        execs.emplace(k, ex);
+        order.push_back(k);
+    }
+};
+static GraphCache g_graphs;
+
 // ---- exclusive scan: out[0..n] with out[n] = total ---------------------------------
 
 __global__ void k_scan_total(const u64* in, u64* out, long long n)
@@ -204,7 +278,34 @@ struct Scanner {
         while ((int)lv.size() <= i) lv.emplace_back(new DBuf<u64>);
         return *lv[i];
     }
-    void run(const u64* in, u64* out, long long n, int level = 0)
+    // the scan through the graph cache
+    void run(const u64* in, u64* out, long long n)
+    {
+        // the temporary levels this n needs, allocated before any capture
+        int nl = 0;
+        for (long long m = n; m > 0;) {
+            const long long nb = (m + 1023) / 1024;
+            tmp(2 * nl).ensure(nb + 1);
+            if (nb > 1) tmp(2 * nl + 1).ensure(nb + 1);
+            m = nb > 1 ? nb : 0;
+            nl++;
+        }
+        GraphKey k{};
+        k.w[0] = 0x5343414eull;   // scan
+        k.w[1] = (uint64_t)in;
+        k.w[2] = (uint64_t)out;
+        k.w[3] = (uint64_t)n;
+        for (int i = 0; i < nl && i < 4; i++) {
+            k.w[4 + 2 * i] = (uint64_t)tmp(2 * i).p;
+            k.w[5 + 2 * i] = (uint64_t)tmp(2 * i + 1).p;
+        }
+        if (nl > 4) {   // (2^60 elements): no cache
+            run_level(in, out, n, 0);
+            return;
+        }
+        g_graphs.run(k, fstream(), 3 * nl + 1, [&] { run_level(in, out, n, 0); });
+    }
+    void run_level(const u64* in, u64* out, long long n, int level)
     {
         hipStream_t st = fstream();
         const long long nb = (n + 1023) / 1024;
@@ -214,7 +315,7 @@ struct Scanner {
             HC(hipGetLastError());
             if (nb > 1) {
                 u64* ss = tmp(2 * level + 1).ensure(nb + 1);
-                run(sums, ss, nb, level + 1);
+                run_level(sums, ss, nb, level + 1);
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(scan_add<u64>), dim3(nb), dim3(256), 0, st, out, n, ss);
                 HC(hipGetLastError());
             }
@@ -239,17 +340,38 @@ static Rec* radix_sort(Rec* a, Rec* b, long long n, int lo, int hi, KeyOf key)
     unsigned* counts = g_rcounts.ensure((size_t)256 * ntiles);
     u64* offs = g_roffs.ensure((size_t)256 * ntiles);
     u64* tot = g_rtot.ensure(256);
-    for (int sh = lo; sh < hi; sh += 8) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_hist<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, n, sh, key, counts, ntiles);
-        HC(hipGetLastError());
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_digit_totals), dim3(256), dim3(256), 0, st, counts, ntiles, tot);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_digit_offsets), dim3(256), dim3(256), 0, st, counts, ntiles, tot, offs);
-        HC(hipGetLastError());
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_scatter<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, a, b, n, sh, key, offs, ntiles);
-        HC(hipGetLastError());
-        std::swap(a, b);
-    }
-    return a;
+    static const int kind = 0;   // one address per Rec / KeyOf instantiation
+    static_assert(sizeof(KeyOf) <= 16, "sort key functor fits the graph key");
+    GraphKey k{};
+    k.w[0] = (uint64_t)&kind;
+    k.w[1] = (uint64_t)a;
+    k.w[2] = (uint64_t)b;
+    k.w[3] = (uint64_t)n;
+    k.w[4] = (uint64_t)lo | ((uint64_t)hi << 32);
+    memcpy(&k.w[5], &key, sizeof(KeyOf));
+    k.w[7] = (uint64_t)counts;
+    k.w[8] = (uint64_t)offs;
+    k.w[9] = (uint64_t)tot;
+    int passes = 0;
+    for (int sh = lo; sh < hi; sh += 8) passes++;
+    Rec* pa = a;
+    Rec* pb = b;
+    g_graphs.run(k, st, 4 * passes, [&] {
+        for (int sh = lo; sh < hi; sh += 8) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_hist<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, pa, n, sh, key,
+                               counts, ntiles);
+            HC(hipGetLastError());
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_digit_totals), dim3(256), dim3(256), 0, st, counts, ntiles, tot);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_digit_offsets), dim3(256), dim3(256), 0, st, counts, ntiles, tot,
+                               offs);
+            HC(hipGetLastError());
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(rs_scatter<Rec, KeyOf>), dim3(ntiles), dim3(256), 0, st, pa, pb, n, sh,
+                               key, offs, ntiles);
+            HC(hipGetLastError());
+            std::swap(pa, pb);
+        }
+    });
+    return (passes & 1) ? b : a;
 }
 
 // ---- k-mer index -----------------------------------------------------------------------
@@ -789,24 +911,30 @@ __global__ __launch_bounds__(64) void k_trace_gather(OutRec* __restrict__ recs, 
     if (threadIdx.x == 0) recs[i].toff = (long long)pos[i];
 }
 
-// candidate scorer state (scorer.hip)
+// candidate scorer state (scorer.hip): mode 0 exact, 1 capture, 2 gate
+constexpr int SC_EXACT = 0, SC_CAPTURE = 1, SC_GATE = 2;
 int scorer_mode();
-const MlpGate* scorer_gate();
-void scorer_capture_append(const float* feat, const int* label, long long n);
+bool scorer_weights(MlpW* w, float* thr);
+void scorer_logits(const float* dfeat, long long n, const MlpW& w, float* dout, hipStream_t st);
+void scorer_capture_append(const float* feat, const int* nrec, const unsigned long long* work, long long n);
+
+// records of each job's final launch, counted per job
+__global__ void k_job_records(const OutRec* __restrict__ r, long long n, int njobs, const int* __restrict__ final_gen,
+                              int* __restrict__ cnt)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const OutRec x = r[i];
+    if (x.job >= 0 && x.job < njobs && final_gen[x.job] == x.gen) atomicAdd(&cnt[x.job], 1);
+}
 
 template <int NCH>
-static void launch_report(const ReportArgs& a, int grid, int sc)
+static void launch_report(const ReportArgs& a, int grid)
 {
-    switch (sc) {
-    case SC_CAPTURE:
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, SC_CAPTURE>), dim3(grid), dim3(64), 0, fstream(), a);
-        break;
-    case SC_GATE:
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, SC_GATE>), dim3(grid), dim3(64), 0, fstream(), a);
-        break;
-    default:
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, SC_EXACT>), dim3(grid), dim3(64), 0, fstream(), a);
-    }
+    if (a.job_work || a.redund)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, true>), dim3(grid), dim3(64), 0, fstream(), a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_report<NCH, false>), dim3(grid), dim3(64), 0, fstream(), a);
 }
 
 static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, SpecObj* spec)
@@ -942,8 +1070,12 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     // aligner windows of 64, 128, 256 and 1,024 diagonals; the last level
     // also has the large pebble pools and match arenas
     constexpr int kLevels = 5;
-    const Level levels[kLevels] = {{1, 1 << 15, 128, 1 << 16, cus * 32},
-                                   {2, 1 << 16, 256, 1 << 18, cus * 16},
+    const MiTuning& tu = mi_tuning();
+    const int l0_waves = cus * (tu.report_waves_per_cu ? tu.report_waves_per_cu : 32);
+    const int l0_pool = 1 << (tu.report_pool_log2 ? tu.report_pool_log2 : 15);
+    const int l1_waves = cus * (tu.report_l1_waves_per_cu ? tu.report_l1_waves_per_cu : 16);
+    const Level levels[kLevels] = {{1, l0_pool, 128, 1 << 16, l0_waves},
+                                   {2, 1 << 16, 256, 1 << 18, l1_waves},
                                    {4, 1 << 17, 256, 1 << 18, cus * 8},
                                    {16, 1 << 17, 256, 1 << 18, cus * 4},
                                    {16, 1 << 20, 4096, 1LL << 22, std::min(cus, 256)}};
@@ -978,6 +1110,10 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     // gate are finished here (no alignment, no record); the rest run
     // longest-first
     long long nfirst = 0;
+    const int sc_mode = scorer_mode();
+    DBuf<float> jfeat;   // scorer features per job (capture / gate)
+    DBuf<u64> jwork;     // capture: aligner work per job
+    DBuf<u64> kflag;     // capture: the jobs past the pre-check
     {
         DBuf<u64> keep, kscan, k1, k2;
         keep.ensure(njobs);
@@ -987,6 +1123,39 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prefilter), dim3(nblk(njobs)), dim3(256), 0, st, jobs.p, (int)njobs,
                            pstart.p, hits, K, BS, g_hitmin, minhit, mindiag, maxdiag, keep.p, touch.p);
         HC(hipGetLastError());
+        // candidate scorer: features of every pair past the pre-check; gating
+        // drops the pairs whose logit is below the threshold before any
+        // alignment, capture keeps the features for the outcome labels
+        if (sc_mode != SC_EXACT) {
+            jfeat.ensure((size_t)njobs * kJobFeat);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_features), dim3((njobs + 127) / 128), dim3(128), 0, st, jobs.p,
+                               (int)njobs, keep.p, pstart.p, hits, A->pk.p, A->dw0.p, A->dlen.p, B->dlen.p, K, BS,
+                               minhit, comp, jfeat.p);
+            HC(hipGetLastError());
+            if (sc_mode == SC_GATE) {
+                MlpW w;
+                float thr;
+                if (!scorer_weights(&w, &thr)) throw Fail("candidate gating requested without scorer weights", MI_ERR_ARG);
+                DBuf<float> lg;
+                DBuf<u64> gc;
+                lg.ensure(njobs);
+                gc.ensure(2);
+                HC(hipMemsetAsync(gc.p, 0, 2 * sizeof(u64), st));
+                scorer_logits(jfeat.p, njobs, w, lg.p, st);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gate_jobs), dim3(nblk(njobs)), dim3(256), 0, st, keep.p, lg.p,
+                                   (int)njobs, thr, gc.p);
+                HC(hipGetLastError());
+                u64 hc[2];
+                HC(hipMemcpy(hc, gc.p, sizeof hc, hipMemcpyDeviceToHost));
+                S.cand_scored += (long long)hc[0];
+                S.cand_skipped += (long long)hc[1];
+            } else {
+                jwork.ensure(njobs);
+                kflag.ensure(njobs);
+                HC(hipMemsetAsync(jwork.p, 0, sizeof(u64) * njobs, st));
+                HC(hipMemcpyAsync(kflag.p, keep.p, sizeof(u64) * njobs, hipMemcpyDeviceToDevice, st));
+            }
+        }
         g_scan.run(keep.p, kscan.p, njobs);
         nfirst = (long long)d2h1(kscan.p + njobs);
         S.njobs_kept += nfirst;
@@ -1049,22 +1218,14 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     a.abort_flag = mi_abort_flag_device();
     a.stats = stats.p;
     a.ocount = ocount.p;
-    // candidate scorer: gate with the loaded weights, or capture features + outcomes
-    const int sc_mode = scorer_mode();
-    if (sc_mode == SC_GATE && (a.gate = scorer_gate()) == nullptr)
-        throw Fail("candidate gating requested without scorer weights", MI_ERR_ARG);
-    DBuf<float> capf;
-    DBuf<int> capm;
-    DBuf<u64> capn;
-    if (sc_mode == SC_CAPTURE) {
-        a.cap_cap = std::max<long long>(1 << 16, 8 * nfirst);
-        capf.ensure((size_t)a.cap_cap * kFeat);
-        capm.ensure((size_t)a.cap_cap * 4);
-        capn.ensure(1);
-        HC(hipMemsetAsync(capn.p, 0, sizeof(u64), st));
-        a.cap_feat = capf.p;
-        a.cap_meta = capm.p;
-        a.cap_count = capn.p;
+    a.job_work = sc_mode == SC_CAPTURE ? jwork.p : nullptr;
+    // MI_REDUND_STATS=1: count the branches handle_redundancies takes (final run of every job)
+    DBuf<unsigned> redund;
+    const bool want_redund = getenv("MI_REDUND_STATS") != nullptr && atoi(getenv("MI_REDUND_STATS")) != 0;
+    if (want_redund) {
+        redund.ensure((size_t)njobs * kRedundBranches);
+        HC(hipMemsetAsync(redund.p, 0, sizeof(unsigned) * njobs * kRedundBranches, st));
+        a.redund = redund.p;
     }
 
     int cur_level = -1, gen = 0;
@@ -1115,10 +1276,10 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         b.otcap = otcap;
         g_ev_report.record_start();
         switch (L.nch) {
-        case 1: launch_report<1>(b, grid, sc_mode); break;
-        case 2: launch_report<2>(b, grid, sc_mode); break;
-        case 4: launch_report<4>(b, grid, sc_mode); break;
-        default: launch_report<16>(b, grid, sc_mode); break;
+        case 1: launch_report<1>(b, grid); break;
+        case 2: launch_report<2>(b, grid); break;
+        case 4: launch_report<4>(b, grid); break;
+        default: launch_report<16>(b, grid); break;
         }
         HC(hipGetLastError());
         g_ev_report.record_stop();
@@ -1286,25 +1447,6 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     // GPU by (aread, bread, abpos, position in the pair's list) -- the order
     // report_thread's buffers give equal (aread, bread, COMP, abpos) keys --
     // compacted with their traces, and copied back in one piece.
-    if (sc_mode == SC_CAPTURE) {
-        // keep the rows of every job's final launch
-        const long long nc = std::min<long long>((long long)d2h1(capn.p), a.cap_cap);
-        std::vector<float> hf((size_t)nc * kFeat);
-        std::vector<int> hm((size_t)nc * 4), hg(njobs);
-        if (nc > 0) {
-            HC(hipMemcpy(hf.data(), capf.p, sizeof(float) * hf.size(), hipMemcpyDeviceToHost));
-            HC(hipMemcpy(hm.data(), capm.p, sizeof(int) * hm.size(), hipMemcpyDeviceToHost));
-            HC(hipMemcpy(hg.data(), final_gen.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
-        }
-        std::vector<float> kf;
-        std::vector<int> kl;
-        for (long long i = 0; i < nc; i++) {
-            if (hg[hm[4 * i + 1]] != hm[4 * i + 2]) continue;
-            kf.insert(kf.end(), hf.begin() + i * kFeat, hf.begin() + (i + 1) * kFeat);
-            kl.push_back(hm[4 * i]);
-        }
-        scorer_capture_append(kf.data(), kl.data(), (long long)kl.size());
-    }
     u64 oc[2], sv[7];
     HC(hipMemcpy(oc, ocount.p, sizeof oc, hipMemcpyDeviceToHost));
     HC(hipMemcpy(sv, stats.p, sizeof sv, hipMemcpyDeviceToHost));
@@ -1313,9 +1455,39 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     S.snake += (long long)sv[2];
     S.edge_events += (long long)sv[3];
     S.wave_steps += (long long)sv[4];
-    S.cand_scored += (long long)sv[5];
-    S.cand_skipped += (long long)sv[6];
     const long long nrec = std::min<long long>((long long)oc[0], ocap);
+    if (want_redund) {
+        std::vector<unsigned> hr((size_t)njobs * kRedundBranches);
+        HC(hipMemcpy(hr.data(), redund.p, sizeof(unsigned) * hr.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < hr.size(); i++) S.redund[i % kRedundBranches] += hr[i];
+    }
+    if (sc_mode == SC_CAPTURE) {
+        // every pair past the pre-check: its features, the records of its final
+        // run and the aligner work that run did
+        DBuf<int> jrec;
+        jrec.ensure(njobs);
+        HC(hipMemsetAsync(jrec.p, 0, sizeof(int) * njobs, st));
+        if (nrec > 0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_job_records), dim3(nblk(nrec)), dim3(256), 0, st, orecs.p, nrec,
+                               (int)njobs, final_gen.p, jrec.p);
+        std::vector<float> hf((size_t)njobs * kJobFeat);
+        std::vector<int> hr(njobs);
+        std::vector<u64> hw(njobs), hk(njobs);
+        HC(hipMemcpy(hf.data(), jfeat.p, sizeof(float) * hf.size(), hipMemcpyDeviceToHost));
+        HC(hipMemcpy(hr.data(), jrec.p, sizeof(int) * njobs, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(hw.data(), jwork.p, sizeof(u64) * njobs, hipMemcpyDeviceToHost));
+        HC(hipMemcpy(hk.data(), kflag.p, sizeof(u64) * njobs, hipMemcpyDeviceToHost));
+        std::vector<float> kf;
+        std::vector<int> kr;
+        std::vector<u64> kw;
+        for (long long j = 0; j < njobs; j++) {
+            if (!hk[j]) continue;
+            kf.insert(kf.end(), hf.begin() + j * kJobFeat, hf.begin() + (j + 1) * kJobFeat);
+            kr.push_back(hr[j]);
+            kw.push_back(hw[j]);
+        }
+        scorer_capture_append(kf.data(), kr.data(), kw.data(), (long long)kr.size());
+    }
     long long nkeep = 0, tbytes_all = 0;
     DBuf<OutRec> srec;
     DBuf<uint8_t> str;
@@ -1463,6 +1635,7 @@ int mi_match_filter(char* aname, HITS_DB* ablock, char* bname, HITS_DB* bblock, 
     (void)bblock;
     memset(&g_stats, 0, sizeof g_stats);
     const double t0 = now_s();
+    const long long gc0 = g_graphs.captured, gr0 = g_graphs.replayed, gl0 = g_graphs.launches_replayed;
     int rc = 0;
     try {
         KIdx* A = as_index(atable);
@@ -1479,6 +1652,9 @@ int mi_match_filter(char* aname, HITS_DB* ablock, char* bname, HITS_DB* bblock, 
     // the B index is consumed when it is not the A index (dalign/filter.c:2194-2202)
     if (btable != atable) mi_free_kmer_index(btable);
     g_stats.t_total = now_s() - t0;
+    g_stats.graphs_captured = g_graphs.captured - gc0;
+    g_stats.graphs_replayed = g_graphs.replayed - gr0;
+    g_stats.launches_replayed = g_graphs.launches_replayed - gl0;
     return rc;
 }
 

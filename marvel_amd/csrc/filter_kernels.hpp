@@ -495,4 +495,139 @@ __global__ void k_prefilter(const int* __restrict__ jobs, int njobs, const long 
     touch[j] = tch ? 1 : 0;
 }
 
+// ---- candidate scorer features (no counterpart in the reference) ------------
+// A candidate is a read pair that passed the gate pre-check (a job of the
+// report kernel).  Its seeds are sorted by apos; one thread streams them into
+// a table of up to kBinSlots diagonal bins (band = diag >> binshift, the
+// report_thread bands), each with its seed count, first and last apos, the
+// sum and sum of squares of the apos spacing of consecutive seeds, and its
+// seeds whose k-mer is a short-period repeat (the 14 bases ending at apos in
+// A repeat with a period of 1..6).  A bin not in a full table replaces the
+// one with the fewest seeds.  The 16 features describe the strongest bin
+// (most seeds), the runner-up and the pair:
+//   0 log2(1 + seeds) / 16            8 log2(1 + alen) / 16
+//   1 seeds in the best bin / seeds   9 log2(1 + blen) / 16
+//   2 log2(1 + best bin seeds) / 16  10 log2(1 + overlap the best bin's diagonal implies) / 16
+//   3 log2(1 + runner-up seeds) / 16 11 best bin apos span / that overlap (capped at 2)
+//   4 best bin apos span / alen      12 COMP
+//   5 log2(1 + best bin span) / 16   13 short-period k-mers in the best bin / its seeds
+//   6 log2(1 + mean spacing) / 16    14 bins with >= minhit seeds / 8
+//   7 log2(1 + spacing sd) / 16      15 short-period k-mers / seeds
+constexpr int kBinSlots = 8;
+constexpr int kJobFeat = 16;
+
+__device__ __forceinline__ bool short_period_kmer(const uint64_t* A, int apos, int K)
+{
+    if (apos < K - 1) return false;
+    // 16 bases ending at apos (base i at bits 2i), 2-bit packed A
+    const uint32_t* a = reinterpret_cast<const uint32_t*>(A);
+    const int p0 = apos - 15;
+    const int q = p0 >> 4;   // p0 >= -15: word q >= -1 is the guard word before the read
+    const uint32_t w = __builtin_amdgcn_alignbit(a[q + 1], a[q], (uint32_t)(2 * (p0 & 15)));
+    const int kk = K < 16 ? K : 16;
+    const uint32_t km = (kk == 16) ? 0xffffffffu : (w >> (2 * (16 - kk)));   // the k-mer's bases, first at bit 0
+    for (int per = 1; per <= 6 && per < kk; per++) {
+        const int nb = kk - per;
+        const uint32_t mask = (nb >= 16) ? 0xffffffffu : ((1u << (2 * nb)) - 1);
+        if (((km >> (2 * per)) & mask) == (km & mask)) return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(128) void k_job_features(const int* __restrict__ jobs, int njobs,
+                                                      const unsigned long long* __restrict__ keep,
+                                                      const long long* __restrict__ pstart, const Seed* __restrict__ h,
+                                                      const uint64_t* __restrict__ pkA, const long long* __restrict__ w0A,
+                                                      const int* __restrict__ lenA, const int* __restrict__ lenB,
+                                                      int K, int binshift, int minhit, int comp,
+                                                      float* __restrict__ feat)
+{
+    __shared__ int tb[128][kBinSlots][5];   // bin, seeds, first apos, last apos, short-period seeds
+    __shared__ float tf[128][kBinSlots][2];  // spacing sum, spacing square sum
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    float* out = feat + (long long)j * kJobFeat;
+    if (!keep[j]) {
+        for (int i = 0; i < kJobFeat; i++) out[i] = 0.f;
+        return;
+    }
+    int (*T)[5] = tb[threadIdx.x];
+    float (*F)[2] = tf[threadIdx.x];
+    int used = 0;
+    const int p = jobs[j];
+    const long long s = pstart[p], e = pstart[p + 1];
+    const int ar = h[s].aread, br = h[s].bread;
+    const int alen = lenA[ar], blen = lenB[br];
+    const uint64_t* A = pkA + w0A[ar];
+    int lc_all = 0;
+    for (long long i = s; i < e; i++) {
+        const Seed x = h[i];
+        const int bin = x.diag >> binshift;
+        const int lc = short_period_kmer(A, x.apos, K) ? 1 : 0;
+        lc_all += lc;
+        int slot = -1, minc = 0x7fffffff, mins = 0;
+        for (int q = 0; q < used; q++) {
+            if (T[q][0] == bin) slot = q;
+            if (T[q][1] < minc) { minc = T[q][1]; mins = q; }
+        }
+        if (slot >= 0) {
+            const float d = (float)(x.apos - T[slot][3]);
+            T[slot][1] += 1;
+            T[slot][3] = x.apos;
+            T[slot][4] += lc;
+            F[slot][0] += d;
+            F[slot][1] += d * d;
+        } else {
+            const int q = used < kBinSlots ? used++ : mins;
+            T[q][0] = bin; T[q][1] = 1; T[q][2] = x.apos; T[q][3] = x.apos; T[q][4] = lc;
+            F[q][0] = 0.f; F[q][1] = 0.f;
+        }
+    }
+    int b1 = 0, b2 = -1, strong = 0;
+    for (int q = 0; q < used; q++) {
+        if (T[q][1] >= minhit) strong++;
+        if (q == 0) continue;
+        if (T[q][1] > T[b1][1]) { b2 = b1; b1 = q; }
+        else if (b2 < 0 || T[q][1] > T[b2][1]) b2 = q;
+    }
+    const float n = (float)(e - s);
+    const float c = (float)T[b1][1];
+    const float span = (float)(T[b1][3] - T[b1][2]);
+    const float mean = c > 1.f ? F[b1][0] / (c - 1.f) : 0.f;
+    const float var = c > 2.f ? fmaxf(F[b1][1] / (c - 1.f) - mean * mean, 0.f) : 0.f;
+    const int dg = T[b1][0] * (1 << binshift) + (1 << binshift) / 2;
+    const int ov = max(min(alen, blen + dg) - max(0, dg), 0);
+    auto lg = [](float v) { return __log2f(1.f + v) * (1.f / 16.f); };
+    out[0] = lg(n);
+    out[1] = c / n;
+    out[2] = lg(c);
+    out[3] = lg(b2 >= 0 ? (float)T[b2][1] : 0.f);
+    out[4] = span / (float)max(alen, 1);
+    out[5] = lg(span);
+    out[6] = lg(mean);
+    out[7] = lg(sqrtf(var));
+    out[8] = lg((float)alen);
+    out[9] = lg((float)blen);
+    out[10] = lg((float)ov);
+    out[11] = fminf(span / (float)max(ov, 1), 2.f);
+    out[12] = (float)comp;
+    out[13] = (float)T[b1][4] / c;
+    out[14] = (float)min(strong, 8) * 0.125f;
+    out[15] = (float)lc_all / n;
+}
+
+// gating: jobs whose logit is below the threshold are not aligned;
+// counts[0] += jobs scored, counts[1] += jobs dropped
+__global__ void k_gate_jobs(unsigned long long* __restrict__ keep, const float* __restrict__ logit, int njobs,
+                            float thr, unsigned long long* __restrict__ counts)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs || !keep[j]) return;
+    atomicAdd(&counts[0], 1ull);
+    if (logit[j] < thr) {
+        keep[j] = 0;
+        atomicAdd(&counts[1], 1ull);
+    }
+}
+
 }  // namespace mi

@@ -239,7 +239,8 @@ static BatchPlan make_plan(int maxlen, int tspace, int cus)
     // a chain holds one pebble per trace point of its read plus the root
     p.pts_cap = maxlen / tspace + 64;
     p.latr_cap = 4 * (2 * (maxlen / tspace + 2));
-    p.lv[0] = Level{1, 1 << 15, cus * 32};   // 8 waves per SIMD
+    const int w0 = mi_tuning().batch_waves_per_cu;
+    p.lv[0] = Level{1, 1 << 15, cus * (w0 ? w0 : 32)};   // default 8 waves per SIMD
     p.lv[1] = Level{4, 1 << 17, cus * 8};
     p.lv[2] = Level{16, 1 << 17, cus * 8};
     p.lv[3] = Level{16, 1 << 20, 256};

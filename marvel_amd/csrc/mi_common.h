@@ -49,6 +49,8 @@ int mi_append_records(Overlap_IO_Buffer* b, const OutRec* recs, long long n, con
 // Device tables for these parameters (cached per parameter set; lives until exit).
 DevSpecHost* mi_spec_device(const MiAlignParams* prm);
 int mi_num_cus();
+// launch shapes chosen by mi_set_tuning (0 fields: built-in defaults)
+const MiTuning& mi_tuning();
 void mi_set_error(const char* msg);
 // Watchdog: device pointer of a host-pinned flag every long-running kernel
 // polls; mi_wait_watchdog waits for stream `st` and, past the deadline

@@ -26,7 +26,6 @@
 #include "filter_kernels.hpp"
 #include "la_wave.hpp"
 #include "records.h"
-#include "scorer.hpp"
 
 namespace mi {
 
@@ -87,20 +86,10 @@ struct ReportArgs {
     int gen;                      // this launch
     int* counter;
     const int* abort_flag;
-    unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events, [4] wave steps,
-                                  // [5] candidates scored, [6] candidates skipped by the scorer
-    // candidate scorer (report kernel instantiations with SC != 0)
-    const MlpGate* gate;          // SC == 2: weights and threshold (align when logit >= thr)
-    float* cap_feat;              // SC == 1: kFeat features per captured candidate
-    int* cap_meta;                //          4 ints: kept, job, launch, 0
-    unsigned long long* cap_count;
-    long long cap_cap;
+    unsigned long long* stats;    // [0] seeds aligned, [1] cells, [2] snake, [3] edge events, [4] wave steps
+    unsigned long long* job_work; // optional, per job: band cells + snake steps of its last successful run
+    unsigned* redund;             // optional, per job: kRedundBranches counters of handle_redundancies
 };
-
-// report kernel variants
-constexpr int SC_EXACT = 0;     // every candidate aligned (the reference's behaviour)
-constexpr int SC_CAPTURE = 1;   // exact, and every aligned candidate's features + outcome recorded
-constexpr int SC_GATE = 2;      // candidates whose scorer logit is below the threshold are not aligned
 
 // ---- redundancy handling (dalign/filter.c:1249-1566), lane 0 ------------------
 
@@ -169,8 +158,15 @@ __device__ bool fusion(MPath* p1, int at, const MPath* p2, uint16_t* T, long lon
     return true;
 }
 
+// Branch counters of handle_redundancies (pass statistics):
+//   0 j starts first, k reaches further: fused, A and B views, COMP      4 j contains k: k replaced
+//   1 the same, not COMP                                                  5 same start, j longer: k replaced
+//   2 the same, A view only                                               6 k starts first, j reaches further: fused, A and B, COMP
+//   3 (unused)                                                            7 the same, not COMP; 8 the same, A view only
+constexpr int kRedundBranches = 9;
+
 __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, long long* top, long long cap, int TS,
-                                   int comp, bool* ok)
+                                   int comp, bool* ok, unsigned (&br)[kRedundBranches])
 {
     int no = 0, j, k, aw = 0, bw = 0;
     for (j = 1; j < n; j++) {
@@ -187,20 +183,24 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
                             *ok &= fusion(jp, aw, kp, T, top, cap, TS);
                             am[k] = *jp;
                             *ok &= fusion(bm + k, bw, bm + j, T, top, cap, TS);
+                            br[0]++;
                         } else {
                             if (entwine(bm + j, bm + k, T, TS, &bw) != 0) continue;
                             *ok &= fusion(jp, aw, kp, T, top, cap, TS);
                             am[k] = *jp;
                             *ok &= fusion(bm + j, bw, bm + k, T, top, cap, TS);
                             bm[k] = bm[j];
+                            br[1]++;
                         }
                     } else {
                         *ok &= fusion(jp, aw, kp, T, top, cap, TS);
                         am[k] = *jp;
+                        br[2]++;
                     }
                 } else {
                     am[k] = *jp;
                     if (bm) bm[k] = bm[j];
+                    br[4]++;
                 }
                 break;
             } else {
@@ -210,6 +210,7 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
                     if (kp->aepos < jp->aepos) {
                         am[k] = *jp;
                         if (bm) bm[k] = bm[j];
+                        br[5]++;
                     }
                 } else if (jp->aepos > kp->aepos) {
                     if (bm) {
@@ -218,13 +219,17 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
                             *ok &= fusion(kp, aw, jp, T, top, cap, TS);
                             *ok &= fusion(bm + j, bw, bm + k, T, top, cap, TS);
                             bm[k] = bm[j];
+                            br[6]++;
                         } else {
                             if (entwine(bm + k, bm + j, T, TS, &bw) != 0) continue;
                             *ok &= fusion(kp, aw, jp, T, top, cap, TS);
                             *ok &= fusion(bm + k, bw, bm + j, T, top, cap, TS);
+                            br[7]++;
                         }
-                    } else
+                    } else {
                         *ok &= fusion(kp, aw, jp, T, top, cap, TS);
+                        br[8]++;
+                    }
                 }
                 break;
             }
@@ -244,8 +249,11 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
 // spills): the aligner is bound by VALU issue and dependent latencies, and
 // measured on the 5 Mbp / 30x workload 3 -> 4 -> 5 -> 6 -> 8 waves per SIMD
 // took the kernel from 380 to 305, 277, 258 and 248 ms per step.
-template <int NCH, int SC>
-__global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report(ReportArgs a)
+// AUX: the instantiation that also fills job_work / redund (capture mode and
+// the branch statistics); the default one keeps those pointers out of its
+// register budget.
+template <int NCH, bool AUX>
+__global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
 {
     const int lane = __lane_id();
     const int wave = blockIdx.x;
@@ -285,8 +293,7 @@ __global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report
         int pstatus = RP_OK;
         int tmin = INT32_MAX, tmax = -INT32_MAX;   // lasta cells raised by this pair
         bool touched = false;
-        long long aligned = 0, edges = 0, scored = 0, skipped = 0;
-        int kept = 0;
+        long long aligned = 0, edges = 0;
         WaveCount wc;
         {
             const int e0 = a.edge_in ? a.edge_in[job] : 0;
@@ -372,29 +379,6 @@ __global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report
                     f0 = fh + 1;
                     const Seed sd = a.hits[fh];
                     const int ap = sd.apos, dg = sd.diag, bp = ap - dg, band0 = dg >> a.binshift;
-                    float feats[kFeat];
-                    if constexpr (SC != SC_EXACT) {
-                        const CandIn ci{score[band0], score[band0 - 1], score[band0 + 1], a.hitmin, pe - ps, fh - ps,
-                                        nidx - lidx, ap, bp, dg, alen, blen, a.comp, (int)aligned, kept,
-                                        lasta[band0]};
-                        cand_features(ci, feats);
-                    }
-                    if constexpr (SC == SC_GATE) {
-                        __shared__ __attribute__((aligned(16))) uint16_t mlp_lds[2 * kMlpLdsHalves];
-                        // the gate is read here (the opaque pointer keeps the loads from
-                        // being hoisted out of the seed loop and held across alignments)
-                        const MlpGate* gp = a.gate;
-                        asm volatile("" : "+v"(gp));
-                        const MlpW w = gp->w;
-                        const f32x4 o = mlp_tile(cand_operand(feats), w, mlp_lds);
-                        const float logit = __builtin_bit_cast(
-                            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, o[0]), 0)) + w.b3;
-                        scored++;
-                        if (logit < gp->thr) {   // predicted not to yield a kept overlap
-                            skipped++;
-                            continue;
-                        }
-                    }
                     // a read against itself (-I, self forward pass: the same bases): the
                     // wave may not reach the main diagonal (Local_Alignment's selfie
                     // bounds, dalign/align.c:1966-1985)
@@ -441,22 +425,7 @@ __global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report
                     for (int i = lane; i <= hi - lo; i += 64)
                         if (r.aepos > lasta[lo + i]) lasta[lo + i] = r.aepos;
                     wave_sync();
-                    if constexpr (SC == SC_CAPTURE) {
-                        const bool kp = (r.aepos - r.abpos) + (r.bepos - r.bbpos) >= a.minover;
-                        if (lane == 0) {
-                            const unsigned long long q = atomicAdd(a.cap_count, 1ull);
-                            if ((long long)q < a.cap_cap) {
-#pragma unroll
-                                for (int i = 0; i < kFeat; i++) a.cap_feat[q * kFeat + i] = feats[i];
-                                a.cap_meta[4 * q] = kp ? 1 : 0;
-                                a.cap_meta[4 * q + 1] = job;
-                                a.cap_meta[4 * q + 2] = a.gen;
-                                a.cap_meta[4 * q + 3] = 0;
-                            }
-                        }
-                    }
                     if ((r.aepos - r.abpos) + (r.bepos - r.bbpos) >= a.minover) {
-                        kept++;
                         const long long need = (doA ? r.atlen : 0) + (doB ? r.btlen : 0);
                         if (novla >= a.mcap || novlb >= a.mcap || ttop + need > a.mtcap) {
                             pstatus = RP_ARENA;
@@ -507,11 +476,16 @@ __global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report
             bool ok = true;
             int na = novla, nb = novlb;
             if (lane == 0) {
+                unsigned br[kRedundBranches] = {};
                 if (na > 1) {
-                    if (nb > 1) na = nb = handle_redundancies(am, na, bm, T, &ttop, a.mtcap, TS, a.comp, &ok);
-                    else na = handle_redundancies(am, na, nullptr, T, &ttop, a.mtcap, TS, a.comp, &ok);
+                    if (nb > 1) na = nb = handle_redundancies(am, na, bm, T, &ttop, a.mtcap, TS, a.comp, &ok, br);
+                    else na = handle_redundancies(am, na, nullptr, T, &ttop, a.mtcap, TS, a.comp, &ok, br);
                 } else if (nb > 1)
-                    nb = handle_redundancies(bm, nb, nullptr, T, &ttop, a.mtcap, TS, a.comp, &ok);
+                    nb = handle_redundancies(bm, nb, nullptr, T, &ttop, a.mtcap, TS, a.comp, &ok, br);
+                if constexpr (AUX) {
+                    if (a.redund)   // the job's last successful run overwrites earlier ones
+                        for (int q = 0; q < kRedundBranches; q++) a.redund[(size_t)job * kRedundBranches + q] = br[q];
+                }
             }
             ok = __builtin_amdgcn_readfirstlane(ok ? 1 : 0) != 0;
             na = __builtin_amdgcn_readfirstlane(na);
@@ -580,9 +554,8 @@ __global__ __launch_bounds__(64, NCH == 1 ? (SC == 2 ? 6 : 8) : 1) void k_report
                 atomicAdd(&a.stats[2], (unsigned long long)wc.snake);
                 atomicAdd(&a.stats[3], (unsigned long long)edges);
                 atomicAdd(&a.stats[4], (unsigned long long)wc.steps);
-                if (SC == SC_GATE) {
-                    atomicAdd(&a.stats[5], (unsigned long long)scored);
-                    atomicAdd(&a.stats[6], (unsigned long long)skipped);
+                if constexpr (AUX) {
+                    if (a.job_work) a.job_work[job] = (unsigned long long)(wc.cells + wc.snake);
                 }
             }
         }

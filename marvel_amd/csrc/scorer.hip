@@ -1,7 +1,10 @@
 // Candidate scorer: weights on the device, capture store, batched MFMA
 // inference (host side and C-ABI).  No counterpart in the reference.
+// A candidate is a read pair past the report gate pre-check; its 16
+// features come from k_job_features (filter_kernels.hpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -21,9 +24,9 @@ struct Scorer {
     void* dev = nullptr;     // packed weights + biases, one allocation
     MlpW w{};
     float thr = 0.f;
-    MlpGate* dgate = nullptr;   // device copy of {w, thr} for the report kernel
     std::vector<float> cap_feat;   // captured candidates (kFeat floats each)
-    std::vector<int> cap_label;
+    std::vector<int> cap_nrec;     // records of the candidate's exact run
+    std::vector<unsigned long long> cap_work;   // band cells + snake steps of that run
 };
 
 Scorer& S()
@@ -91,18 +94,30 @@ __global__ __launch_bounds__(256) void k_mlp_batch(const float* __restrict__ fea
 
 int scorer_mode() { return S().mode; }
 
-const MlpGate* scorer_gate()
+bool scorer_weights(MlpW* w, float* thr)
 {
     Scorer& s = S();
-    return s.loaded ? s.dgate : nullptr;
+    if (!s.loaded) return false;
+    *w = s.w;
+    *thr = s.thr;
+    return true;
 }
 
-void scorer_capture_append(const float* feat, const int* label, long long n)
+void scorer_logits(const float* dfeat, long long n, const MlpW& w, float* dout, hipStream_t st)
+{
+    if (n <= 0) return;
+    const long long ntile = (n + kMlpRows - 1) / kMlpRows;
+    const int grid = (int)std::min<long long>((ntile + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_mlp_batch, dim3(grid), dim3(256), 0, st, dfeat, n, w, dout);
+}
+
+void scorer_capture_append(const float* feat, const int* nrec, const unsigned long long* work, long long n)
 {
     Scorer& s = S();
     std::lock_guard<std::mutex> g(s.mu);
     s.cap_feat.insert(s.cap_feat.end(), feat, feat + n * kFeat);
-    s.cap_label.insert(s.cap_label.end(), label, label + n);
+    s.cap_nrec.insert(s.cap_nrec.end(), nrec, nrec + n);
+    s.cap_work.insert(s.cap_work.end(), work, work + n);
 }
 
 }  // namespace mi
@@ -144,12 +159,6 @@ int mi_scorer_load(const float* w1, const float* b1, const float* w2, const floa
     s.w.b2 = s.w.b1 + kHid;
     s.w.b3 = b3;
     s.thr = threshold;
-    const MlpGate g{s.w, threshold};
-    if ((s.dgate == nullptr && hipMalloc(&s.dgate, sizeof(MlpGate)) != hipSuccess) ||
-        hipMemcpy(s.dgate, &g, sizeof g, hipMemcpyHostToDevice) != hipSuccess) {
-        mi_set_error("mi_scorer_load: copy to the device failed");
-        return MI_ERR_DEVICE;
-    }
     s.loaded = true;
     return 0;
 }
@@ -171,20 +180,22 @@ int mi_scorer_mode(int mode)
 long long mi_scorer_captured(void)
 {
     std::lock_guard<std::mutex> g(S().mu);
-    return (long long)S().cap_label.size();
+    return (long long)S().cap_nrec.size();
 }
 
-long long mi_scorer_take(float* feat, int* label, long long cap)
+long long mi_scorer_take(float* feat, int* nrec, unsigned long long* work, long long cap)
 {
     Scorer& s = S();
     std::lock_guard<std::mutex> g(s.mu);
-    const long long n = std::min<long long>(cap, (long long)s.cap_label.size());
+    const long long n = std::min<long long>(cap, (long long)s.cap_nrec.size());
     if (n > 0) {
         memcpy(feat, s.cap_feat.data(), sizeof(float) * n * kFeat);
-        memcpy(label, s.cap_label.data(), sizeof(int) * n);
+        memcpy(nrec, s.cap_nrec.data(), sizeof(int) * n);
+        if (work) memcpy(work, s.cap_work.data(), sizeof(unsigned long long) * n);
     }
     s.cap_feat.clear();
-    s.cap_label.clear();
+    s.cap_nrec.clear();
+    s.cap_work.clear();
     return n;
 }
 

@@ -1,18 +1,15 @@
-// Candidate scorer for report_thread's alignment candidates (no counterpart
-// in the reference).
+// Candidate scorer MLP (no counterpart in the reference).
 //
-// A candidate is a seed that passes report_thread's gate
-// (dalign/filter.c:1772-1778), i.e. a point where the reference calls
-// Local_Alignment.  Sixteen features of the candidate feed an MLP
-// 16 -> 64 -> 64 -> 1 with ReLU hidden layers; bf16 weights and activations,
-// fp32 accumulation.  The logit estimates whether the alignment will produce
-// a kept overlap ((aepos-abpos)+(bepos-bbpos) >= 2*minlen).
+// A candidate is a read pair that passes report_thread's gate pre-check
+// (dalign/filter.c:1740-1778); sixteen features of it (k_job_features) feed
+// an MLP 16 -> 64 -> 64 -> 1 with ReLU hidden layers; bf16 weights and
+// activations, fp32 accumulation.  The logit estimates whether the pair's
+// exact run writes records.
 //
 // The three layers run on gfx950's v_mfma_f32_16x16x32_bf16 over a tile of
 // 16 candidates (rows), one 16-column block at a time, so a wave holds a
-// single 4-float accumulator (the report kernel evaluates the scorer between
-// alignments and keeps its register budget).  Operand layout of that
-// instruction, lane l of the wave:
+// single 4-float accumulator.  Operand layout of that instruction, lane l of
+// the wave:
 //   A (16 x 32): row l & 15, k = 8*(l >> 4) + e, e = 0..7
 //   B (32 x 16): column l & 15, k = 8*(l >> 4) + e
 //   C (16 x 16): column l & 15, rows 4*(l >> 4) + v, v = 0..3
@@ -124,66 +121,6 @@ __device__ __forceinline__ f32x4 mlp_tile(bf16x8 x, const MlpW& w, uint16_t* lds
     for (int t = 0; t < 2; t++) o = mfma16(mlp_load_a(h2, t), w.w3[t * 64 + l], o);
     mlp_wave_sync();
     return o;
-}
-
-// The 16 features of a candidate seed; every input is wave-uniform.
-struct CandIn {
-    int sb, sm, sp;          // score[band], score[band-1], score[band+1]
-    int hitmin;
-    long long nseeds;        // seeds of the pair
-    long long idx;           // seed index within the pair
-    long long panel_n;       // seeds of the current panel
-    int ap, bp, dg;          // seed A / B position, diagonal
-    int alen, blen, comp;
-    int aligned, kept;       // alignments and kept alignments of the pair so far
-    int lasta_b;             // lasta of the seed's band
-};
-
-__device__ __forceinline__ float lg16(float v) { return __log2f(1.f + v) * (1.f / 16.f); }
-
-__device__ __forceinline__ void cand_features(const CandIn& c, float (&f)[kFeat])
-{
-    const float h = (float)c.hitmin;
-    const int side = c.sp > c.sm ? c.sp : c.sm;
-    const int ov = min(c.alen, c.blen + c.dg) - max(0, c.dg);   // overlap length implied by the diagonal
-    const int end = min(min(c.ap, c.alen - c.ap), min(c.bp, c.blen - c.bp));
-    f[0] = (float)(c.sb + side) / h;
-    f[1] = (float)c.sb / h;
-    f[2] = (float)side / h;
-    f[3] = lg16((float)c.nseeds);
-    f[4] = (float)c.idx / (float)c.nseeds;
-    f[5] = (float)c.ap / (float)c.alen;
-    f[6] = (float)c.bp / (float)c.blen;
-    f[7] = lg16((float)c.alen);
-    f[8] = lg16((float)c.blen);
-    f[9] = (float)c.comp;
-    f[10] = lg16((float)max(ov, 0));
-    f[11] = (float)min(c.aligned, 16) * 0.125f;
-    f[12] = (float)min(c.kept, 16) * 0.125f;
-    f[13] = c.lasta_b > 0 ? 1.f : 0.f;
-    f[14] = (float)c.panel_n / (float)c.nseeds;
-    f[15] = lg16((float)max(end, 0));
-}
-
-// Logit of one candidate (row 0 of a tile).  Not inlined: the report
-// kernel's register allocation around the aligner stays free of the MLP's
-// temporaries.  All 64 lanes must be active.
-
-// Weights and threshold of the gate, in device memory: the report kernel
-// reads them where it scores a candidate instead of keeping them live.
-struct MlpGate {
-    MlpW w;
-    float thr;
-};
-
-// Layer-1 A operand of a tile whose row 0 is this candidate (other rows zero).
-__device__ __forceinline__ bf16x8 cand_operand(const float (&f)[kFeat])
-{
-    const int l = __lane_id();
-    bf16x8 r = {};
-    if (l == 0) r = pack8(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]);
-    if (l == 16) r = pack8(f[8], f[9], f[10], f[11], f[12], f[13], f[14], f[15]);
-    return r;
 }
 
 }  // namespace mi

@@ -133,6 +133,10 @@ int mi_wait_watchdog(hipStream_t st, const char* what)
     return 0;
 }
 
+// Launch shapes picked by the tuner (0: built-in default); see MiTuning.
+static MiTuning g_tuning = {0, 0, 0, 0};
+const MiTuning& mi_tuning() { return g_tuning; }
+
 static thread_local std::string g_err;
 void mi_set_error(const char* msg) { g_err = msg; }
 
@@ -182,6 +186,44 @@ int mi_device_memset(void* p, int value, long long bytes)
         mi::mi_set_error("mi_device_memset: failed");
         return MI_ERR_DEVICE;
     }
+    return 0;
+}
+
+int mi_set_tuning(const MiTuning* t)
+{
+    if (t == nullptr) {
+        mi::g_tuning = MiTuning{0, 0, 0, 0};
+        return 0;
+    }
+    const bool ok = (t->report_waves_per_cu == 0 || (t->report_waves_per_cu >= 4 && t->report_waves_per_cu <= 32)) &&
+                    (t->report_pool_log2 == 0 || (t->report_pool_log2 >= 12 && t->report_pool_log2 <= 18)) &&
+                    (t->report_l1_waves_per_cu == 0 || (t->report_l1_waves_per_cu >= 2 && t->report_l1_waves_per_cu <= 32)) &&
+                    (t->batch_waves_per_cu == 0 || (t->batch_waves_per_cu >= 4 && t->batch_waves_per_cu <= 32));
+    if (!ok) {
+        mi::mi_set_error("mi_set_tuning: value out of range");
+        return MI_ERR_ARG;
+    }
+    mi::g_tuning = *t;
+    return 0;
+}
+
+int mi_get_tuning(MiTuning* t)
+{
+    if (t == nullptr) return MI_ERR_ARG;
+    *t = mi::g_tuning;
+    return 0;
+}
+
+int mi_device_info(char* name, int cap, int* cus)
+{
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return MI_ERR_DEVICE;
+    if (name && cap > 0) {
+        strncpy(name, p.gcnArchName, cap - 1);
+        name[cap - 1] = 0;
+    }
+    if (cus) *cus = p.multiProcessorCount;
     return 0;
 }
 

@@ -56,10 +56,14 @@ class MiFilterStats(C.Structure):
                 ("limit", C.c_int), ("t_filter", C.c_double), ("t_report", C.c_double), ("t_copy", C.c_double),
                 ("t_total", C.c_double), ("t_report_kernel", C.c_double), ("report_launches", C.c_longlong),
                 ("t_seed_sort", C.c_double), ("seed_sort_passes", C.c_longlong), ("njobs_kept", C.c_longlong),
-                ("wave_steps", C.c_longlong), ("cand_scored", C.c_longlong), ("cand_skipped", C.c_longlong)]
+                ("wave_steps", C.c_longlong), ("cand_scored", C.c_longlong), ("cand_skipped", C.c_longlong),
+                ("redund", C.c_longlong * 9), ("graphs_captured", C.c_longlong), ("graphs_replayed", C.c_longlong),
+                ("launches_replayed", C.c_longlong)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        v = {k: getattr(self, k) for k, _ in self._fields_}
+        v["redund"] = list(v["redund"])
+        return v
 
 
 assert C.sizeof(HITS_READ) == 32 and C.sizeof(HITS_DB) == 88

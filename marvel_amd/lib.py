@@ -36,6 +36,11 @@ class MiGenTaskParams(C.Structure):
                 ("emax_ppm", C.c_int), ("offmax", C.c_int)]
 
 
+class MiTuning(C.Structure):
+    _fields_ = [("report_waves_per_cu", C.c_int), ("report_pool_log2", C.c_int), ("report_l1_waves_per_cu", C.c_int),
+                ("batch_waves_per_cu", C.c_int)]
+
+
 class MiTraceTask(C.Structure):
     _fields_ = [("aword", C.c_longlong), ("bword", C.c_longlong), ("alen", C.c_int), ("blen", C.c_int),
                 ("abpos", C.c_int), ("bbpos", C.c_int), ("aepos", C.c_int), ("bepos", C.c_int),
@@ -53,7 +58,7 @@ EXPORTS = [
     "Read_Overlap", "Read_Trace", "Write_Overlap", "Compress_TraceTo8", "Decompress_TraceTo16",
     "Check_Trace_Points",
     "mi_la_batch_host", "mi_la_batch_device", "mi_la_batch_device_noalloc", "mi_gen_la_tasks", "mi_abort_device_work", "mi_device_alloc", "mi_device_free",
-    "mi_device_copy", "mi_device_memset", "mi_align_params", "mi_last_error", "mi_device_count", "mi_selftest",
+    "mi_device_copy", "mi_device_memset", "mi_set_tuning", "mi_get_tuning", "mi_device_info", "mi_align_params", "mi_last_error", "mi_device_count", "mi_selftest",
 ]
 
 
@@ -85,6 +90,9 @@ class _Lib:
             h.mi_device_free.argtypes = [C.c_void_p]
             h.mi_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int]
             h.mi_device_memset.argtypes = [C.c_void_p, C.c_int, C.c_longlong]
+            h.mi_set_tuning.argtypes = [C.c_void_p]
+            h.mi_get_tuning.argtypes = [C.c_void_p]
+            h.mi_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
             h.mi_align_params.argtypes = [C.POINTER(MiAlignParams), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]
             h.mi_last_error.restype = C.c_char_p

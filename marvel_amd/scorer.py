@@ -1,17 +1,27 @@
 """Learned candidate scorer for the daligner report path (no reference counterpart).
 
-A candidate is a seed that passes report_thread's band gate
-(dalign/filter.c:1772-1778): the point where the reference calls
-Local_Alignment.  The GPU report kernel computes 16 features per candidate
-(FEATURES) and, in gating mode, evaluates an MLP 16 -> 64 -> 64 -> 1 (ReLU,
-bf16 weights and activations, fp32 accumulation, gfx950 MFMA) and skips the
-alignment when the logit is below the threshold.  Label 1 = the alignment
-from that candidate was kept ((aepos-abpos)+(bepos-bbpos) >= 2*minlen).
+A candidate is a read pair that passes the exact pre-check of
+report_thread's band gate (dalign/filter.c:1740-1778): a pair the plain path
+would hand to the report kernel.  k_job_features (filter_kernels.hpp)
+computes 16 features per candidate from its seeds (FEATURES): the seed
+count and the strongest diagonal bins -- seeds, apos span, mean and sd of the
+seed spacing, the overlap the bin's diagonal implies -- the read lengths and
+the share of short-period (tandem-repeat) k-mers.  In gating mode an MLP
+16 -> 64 -> 64 -> 1 (ReLU, bf16 weights and activations, fp32 accumulation,
+gfx950 MFMA) scores every candidate before alignment and the pairs below the
+threshold are dropped; every pair it keeps runs the exact report kernel.
+
+Labels come from the plain path itself (capture mode): the records each
+pair's exact run writes and the aligner work (band cells + snake steps) it
+takes.  Training runs on the GPU (mi_scorer_train: bf16 MFMA forward and
+backward kernels, fp32 Adam): positives weighted by their records, negatives
+by their aligner work, and the threshold is the highest that keeps the
+target share of the training records.
 
 Workflow:
-    python -m marvel_amd.scorer capture --config small_repeats --out X.npz   (GPU: exact run + features)
-    python -m marvel_amd.scorer train X1.npz X2.npz --out W.json          (CPU: torch training)
-    python -m marvel_amd.scorer evaluate --config small --weights W.json  (GPU: record recall of gating)
+    python -m marvel_amd.scorer capture --config 5mbp --out X.npz          (GPU: exact run + features)
+    python -m marvel_amd.scorer train X1.npz X2.npz --out W.json           (GPU training)
+    python -m marvel_amd.scorer evaluate --config repeats_50mbp            (GPU: record recall and time of gating)
 
 The exact path (mode 0) is untouched by all of this; gating trades recall for
 fewer alignments and is reported separately.
@@ -34,23 +44,24 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "data", "scorer_weights.json")
 NFEAT, NHID = 16, 64
+NPARAM = NFEAT * NHID + NHID + NHID * NHID + NHID + NHID + 1   # w1, b1, w2, b2, w3, b3
 FEATURES = [
-    "gate score (score[b] + max(score[b-1], score[b+1])) / h",
-    "score[b] / h",
-    "max(score[b-1], score[b+1]) / h",
     "log2(1 + seeds of the pair) / 16",
-    "seed index within the pair / seeds of the pair",
-    "apos / alen",
-    "bpos / blen",
+    "seeds in the strongest diagonal bin / seeds",
+    "log2(1 + seeds in the strongest bin) / 16",
+    "log2(1 + seeds in the runner-up bin) / 16",
+    "apos span of the strongest bin / alen",
+    "log2(1 + apos span of the strongest bin) / 16",
+    "log2(1 + mean apos spacing of its seeds) / 16",
+    "log2(1 + sd of that spacing) / 16",
     "log2(1 + alen) / 16",
     "log2(1 + blen) / 16",
-    "comp",
-    "log2(1 + overlap length implied by the diagonal) / 16",
-    "alignments of the pair so far / 8 (capped at 16)",
-    "kept alignments of the pair so far / 8 (capped at 16)",
-    "lasta[b] > 0",
-    "seeds of the panel / seeds of the pair",
-    "log2(1 + distance of the seed to the nearest read end) / 16",
+    "log2(1 + overlap implied by the bin's diagonal) / 16",
+    "apos span / that overlap (capped at 2)",
+    "COMP",
+    "short-period k-mers in the strongest bin / its seeds",
+    "bins with >= minhit seeds / 8 (8 bins tracked)",
+    "short-period k-mers / seeds",
 ]
 
 
@@ -99,52 +110,58 @@ def reference_logits(X: np.ndarray, W: ScorerWeights, kernel_rounding: bool = Tr
     return (h2 @ r(W.w3) + np.float32(W.b3)).astype(np.float32)
 
 
-def train(X: np.ndarray, y: np.ndarray, epochs: int = 40, seed: int = 0, lr: float = 2e-3,
-          batch: int = 2048) -> ScorerWeights:
-    """Trains the MLP with torch on the CPU (binary cross-entropy with the
-    classes balanced).  Threshold left at 0; see choose_threshold."""
-    import torch
+def sample_weights(nrec: np.ndarray, work: np.ndarray) -> np.ndarray:
+    """Training weights of captured pairs: a pair with records weighs its
+    record count (recall is counted in records), a pair without weighs the
+    aligner work it took (what dropping it saves); each class sums to half
+    the number of rows."""
+    nrec = np.asarray(nrec, dtype=np.float64)
+    work = np.asarray(work, dtype=np.float64)
+    pos = nrec > 0
+    w = np.zeros(len(nrec))
+    n = len(nrec)
+    if pos.any():
+        w[pos] = nrec[pos] / nrec[pos].sum() * (n / 2)
+    if (~pos).any():
+        wk = np.maximum(work[~pos], 1.0)
+        w[~pos] = wk / wk.sum() * (n / 2)
+    return w.astype(np.float32)
 
-    torch.manual_seed(seed)
+
+def init_params(seed: int) -> np.ndarray:
+    """He-initialised weights and zero biases as a parameter vector."""
     rng = np.random.default_rng(seed)
-    Xt = torch.from_numpy(X.astype(np.float32))
-    yt = torch.from_numpy(y.astype(np.float32))
-    model = torch.nn.Sequential(torch.nn.Linear(NFEAT, NHID), torch.nn.ReLU(), torch.nn.Linear(NHID, NHID),
-                                torch.nn.ReLU(), torch.nn.Linear(NHID, 1))
-    npos = float(y.sum())
-    nneg = float(len(y) - npos)
-    pos_weight = torch.tensor([nneg / max(npos, 1.0)])
-    lossf = torch.nn.BCEWithLogitsLoss(pos_weight=pos_weight)
-    opt = torch.optim.Adam(model.parameters(), lr=lr)
-    n = len(y)
-    for ep in range(epochs):
-        perm = torch.from_numpy(rng.permutation(n))
-        for i in range(0, n, batch):
-            idx = perm[i:i + batch]
-            opt.zero_grad()
-            loss = lossf(model(Xt[idx]).squeeze(1), yt[idx])
-            loss.backward()
-            opt.step()
-    W = weights_from_torch(model)
-    W.meta = {"epochs": epochs, "seed": seed, "n": int(n), "pos": int(npos)}
-    return W
+    W = ScorerWeights(rng.normal(0, np.sqrt(2 / NFEAT), (NFEAT, NHID)).astype(np.float32), np.zeros(NHID, np.float32),
+                      rng.normal(0, np.sqrt(2 / NHID), (NHID, NHID)).astype(np.float32), np.zeros(NHID, np.float32),
+                      rng.normal(0, np.sqrt(1 / NHID), NHID).astype(np.float32), 0.0, 0.0)
+    return params_vector(W)
 
 
-def weights_from_torch(model) -> ScorerWeights:
-    """ScorerWeights of a torch Sequential(Linear(16,64), ReLU, Linear(64,64), ReLU, Linear(64,1))."""
-    l1, l2, l3 = model[0], model[2], model[4]
-    g = lambda t: t.detach().numpy().astype(np.float32)
-    return ScorerWeights(g(l1.weight).T.copy(), g(l1.bias), g(l2.weight).T.copy(), g(l2.bias), g(l3.weight)[0].copy(),
-                         float(g(l3.bias)[0]), 0.0)
+def params_vector(W: ScorerWeights) -> np.ndarray:
+    """The parameter vector of the GPU trainer: w1, b1, w2, b2, w3, b3."""
+    return np.concatenate([np.ravel(W.w1), W.b1, np.ravel(W.w2), W.b2, W.w3, [W.b3]]).astype(np.float32)
+
+
+def weights_from_vector(p: np.ndarray, threshold: float = 0.0) -> ScorerWeights:
+    o = [0, 1024, 1088, 5184, 5248, 5312]
+    return ScorerWeights(p[o[0]:o[1]].reshape(NFEAT, NHID).copy(), p[o[1]:o[2]].copy(),
+                         p[o[2]:o[3]].reshape(NHID, NHID).copy(), p[o[3]:o[4]].copy(), p[o[4]:o[5]].copy(),
+                         float(p[o[5]]), threshold)
 
 
 def choose_threshold(logits: np.ndarray, y: np.ndarray, recall: float) -> float:
-    """Largest threshold that keeps at least `recall` of the positives."""
-    pos = np.sort(logits[y == 1])
-    if len(pos) == 0:
+    """Largest threshold that keeps at least `recall` of the positives'
+    weight: y is 0 / 1 or a record count per row (logit >= threshold keeps)."""
+    y = np.asarray(y, dtype=np.float64)
+    pos = y > 0
+    if not pos.any():
         return float("-inf")
-    k = int(np.floor((1.0 - recall) * len(pos)))   # positives that may fall below
-    return float(np.nextafter(pos[k], -np.inf))
+    order = np.argsort(logits[pos], kind="stable")
+    lg, wt = logits[pos][order], y[pos][order]
+    total = wt.sum()
+    lost = np.concatenate([[0.0], np.cumsum(wt)])   # weight below index k
+    k = int(np.flatnonzero(total - lost[:-1] >= recall * total - 1e-9 * total).max())
+    return float(np.nextafter(np.float32(lg[k]), np.float32(-np.inf)))
 
 
 # ---- GPU binding ---------------------------------------------------------------
@@ -155,8 +172,10 @@ def _bind(h):
         "mi_scorer_load": ([fp, fp, fp, fp, fp, C.c_float, C.c_float], C.c_int),
         "mi_scorer_mode": ([C.c_int], C.c_int),
         "mi_scorer_captured": ([], C.c_longlong),
-        "mi_scorer_take": ([fp, C.POINTER(C.c_int), C.c_longlong], C.c_longlong),
+        "mi_scorer_take": ([fp, C.POINTER(C.c_int), C.POINTER(C.c_ulonglong), C.c_longlong], C.c_longlong),
         "mi_mlp_score": ([fp, C.c_longlong, fp], C.c_int),
+        "mi_scorer_grad": ([fp, fp, fp, C.c_longlong, fp, fp, fp], C.c_int),
+        "mi_scorer_train": ([fp, fp, fp, C.c_longlong, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp, fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(h, name)
@@ -191,11 +210,38 @@ class GpuScorer:
         self._check(self.h.mi_scorer_mode(m), "mi_scorer_mode")
 
     def take(self):
+        """Captured rows: (features, records of the exact run, aligner work)."""
         n = self.h.mi_scorer_captured()
         X = np.zeros((max(n, 1), NFEAT), dtype=np.float32)
-        y = np.zeros(max(n, 1), dtype=np.int32)
-        got = self.h.mi_scorer_take(_fp(X), y.ctypes.data_as(C.POINTER(C.c_int)), n)
-        return X[:got], y[:got]
+        nrec = np.zeros(max(n, 1), dtype=np.int32)
+        work = np.zeros(max(n, 1), dtype=np.uint64)
+        got = self.h.mi_scorer_take(_fp(X), nrec.ctypes.data_as(C.POINTER(C.c_int)),
+                                    work.ctypes.data_as(C.POINTER(C.c_ulonglong)), n)
+        return X[:got], nrec[:got], work[:got]
+
+    def grad(self, X: np.ndarray, y: np.ndarray, w: np.ndarray, params: np.ndarray):
+        """Loss gradient over all rows at `params` (5313 floats) on the GPU: (grad, loss)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.zeros(NPARAM, dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(self.h.mi_scorer_grad(_fp(X), _fp(y), _fp(w), len(y), _fp(p), _fp(g), C.byref(loss)),
+                    "mi_scorer_grad")
+        return g, float(loss.value)
+
+    def train(self, X: np.ndarray, y: np.ndarray, w: np.ndarray, params: np.ndarray, epochs: int = 30,
+              batch: int = 4096, lr: float = 2e-3, seed: int = 0):
+        """Adam on the GPU from `params`; returns (trained params, loss per epoch)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        p = np.array(params, dtype=np.float32)
+        el = np.zeros(epochs, dtype=np.float32)
+        self._check(self.h.mi_scorer_train(_fp(X), _fp(y), _fp(w), len(y), epochs, batch, C.c_float(lr), seed,
+                                           _fp(p), _fp(el)), "mi_scorer_train")
+        return p, el
 
     def logits(self, X: np.ndarray) -> np.ndarray:
         X = np.ascontiguousarray(X, dtype=np.float32)
@@ -243,22 +289,46 @@ def _reads(config: str):
         return sim.config_small_repeats(12).reads
     if config == "5mbp":
         return sim.config_5mbp(2).reads
+    if config == "repeats_50mbp":   # the first 200 Mbp block of the 50 Mbp repeat genome at 30x, 12 % error
+        return sim.config_repeats_50mbp(4).reads
     raise ValueError(f"unknown config {config}")
+
+
+def train_gpu(sc: GpuScorer, X: np.ndarray, nrec: np.ndarray, work: np.ndarray, recall: float = 0.999,
+              epochs: int = 40, batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> ScorerWeights:
+    """Trains the MLP on the GPU from captured rows (sample_weights) and sets
+    the threshold that keeps `recall` of the training records."""
+    w = sample_weights(nrec, work)
+    y = (np.asarray(nrec) > 0).astype(np.float32)
+    p, losses = sc.train(X, y, w, init_params(seed), epochs=epochs, batch=batch, lr=lr, seed=seed)
+    W = weights_from_vector(p)
+    sc.load(W)
+    lg = sc.logits(X)
+    W.threshold = choose_threshold(lg, nrec, recall)
+    keep = lg >= W.threshold
+    work = np.asarray(work, dtype=np.float64)
+    W.meta = {"epochs": epochs, "seed": seed, "n": int(len(y)), "pos": int(y.sum()), "recall_target": recall,
+              "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+              "train_record_recall": float(np.asarray(nrec)[keep].sum() / max(1, np.asarray(nrec).sum())),
+              "train_pairs_dropped": float(1 - keep.mean()),
+              "train_work_dropped": float(work[~keep].sum() / max(1.0, work.sum()))}
+    return W
 
 
 def run_self(config: str, mode: int, W: ScorerWeights | None = None, workdir: str | None = None,
              mem_gb: int | None = None):
     """One block of `config` against itself (daligner -k14 -w6 -h35 -e.70
-    -l1000 -s100 -j8, -M16 for the small configs and -M32 for 5mbp) with
-    scorer mode 0 / 1 / 2.  Returns (las path, (forward stats, complement
-    stats), captured X, y)."""
+    -l1000 -s100 -j8, -M16 for the small configs, -M32 otherwise) with scorer
+    mode 0 / 1 / 2.  Returns (las path, (forward stats, complement stats),
+    captured (X, nrec, work) or None)."""
     from . import dbio
     from .daligner import Daligner
     d = workdir or tempfile.mkdtemp(prefix="miscorer_", dir="/dev/shm" if os.access("/dev/shm", os.W_OK) else None)
+    big = config in ("5mbp", "repeats_50mbp")
     dbio.write_db(os.path.join(d, "S"), _reads(config), size_mbp=400)
     dal = Daligner()
     sc = GpuScorer(dal.h)
-    mem = mem_gb if mem_gb is not None else (32 if config == "5mbp" else 16)
+    mem = mem_gb if mem_gb is not None else (32 if big else 16)
     dal.configure(nthreads=8, mem_limit_gb=mem, minover=2000, symmetric=1)
     dal.set_filter_params(14, 6, 0, 35)
     A = dal.open_block(os.path.join(d, "S.1"))
@@ -273,57 +343,58 @@ def run_self(config: str, mode: int, W: ScorerWeights | None = None, workdir: st
     finally:
         sc.mode(0)
         os.chdir(cwd)
-    X, y = sc.take() if mode == 1 else (None, None)
-    return os.path.join(d, "001_00001", "S.1.S.1.las"), stats, X, y
+    cap = sc.take() if mode == 1 else None
+    return os.path.join(d, "001_00001", "S.1.S.1.las"), stats, cap
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m marvel_amd.scorer")
     sub = ap.add_subparsers(dest="cmd", required=True)
-    c = sub.add_parser("capture", help="exact GPU run recording candidate features and outcomes")
-    c.add_argument("--config", default="small_repeats")
+    c = sub.add_parser("capture", help="exact GPU run recording every candidate's features and outcome")
+    c.add_argument("--config", default="5mbp")
     c.add_argument("--out", required=True)
-    t = sub.add_parser("train", help="train on captured .npz files (CPU)")
+    t = sub.add_parser("train", help="train on captured .npz files (GPU)")
     t.add_argument("inputs", nargs="+")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
-    t.add_argument("--recall", type=float, default=0.999, help="candidate-level recall the threshold keeps")
+    t.add_argument("--recall", type=float, default=0.999, help="share of the training records the threshold keeps")
     t.add_argument("--epochs", type=int, default=40)
-    e = sub.add_parser("evaluate", help="record recall and skipped alignments of gating (GPU)")
-    e.add_argument("--config", default="small")
+    e = sub.add_parser("evaluate", help="record recall and time of gating (GPU)")
+    e.add_argument("--config", default="repeats_50mbp")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
     if a.cmd == "capture":
-        las, stats, X, y = run_self(a.config, 1)
-        np.savez_compressed(a.out, X=X, y=y, config=np.array(a.config))
-        print(json.dumps({"config": a.config, "candidates": int(len(y)), "kept": int(y.sum())}))
+        las, stats, (X, nrec, work) = run_self(a.config, 1)
+        np.savez_compressed(a.out, X=X, nrec=nrec, work=work, config=np.array(a.config))
+        print(json.dumps({"config": a.config, "candidates": int(len(nrec)), "with_records": int((nrec > 0).sum()),
+                          "records": int(nrec.sum()), "work": int(work.sum())}))
         shutil.rmtree(os.path.dirname(os.path.dirname(las)), ignore_errors=True)
     elif a.cmd == "train":
-        Xs, ys, cfgs = [], [], []
+        from .daligner import Daligner
+        Xs, rs, ws, cfgs = [], [], [], []
         for p in a.inputs:
             z = np.load(p)
             Xs.append(z["X"])
-            ys.append(z["y"])
+            rs.append(z["nrec"])
+            ws.append(z["work"])
             cfgs.append(str(z["config"]))
-        X, y = np.concatenate(Xs), np.concatenate(ys)
-        W = train(X, y, epochs=a.epochs)
-        lg = reference_logits(X, W)
-        W.threshold = choose_threshold(lg, y, a.recall)
-        keep = lg >= W.threshold
-        W.meta.update(recall_target=a.recall, configs=cfgs,
-                      train_recall=float(keep[y == 1].mean()), train_skip=float(1 - keep.mean()),
-                      neg_skipped=float(1 - keep[y == 0].mean()) if (y == 0).any() else None)
+        sc = GpuScorer(Daligner().h)
+        W = train_gpu(sc, np.concatenate(Xs), np.concatenate(rs), np.concatenate(ws), recall=a.recall,
+                      epochs=a.epochs)
+        W.meta["configs"] = cfgs
         W.save(a.out)
         print(json.dumps(W.meta))
     else:
         W = ScorerWeights.load(a.weights)
-        ex, s_ex, _, _ = run_self(a.config, 0)
-        gd, s_gd, _, _ = run_self(a.config, 2, W)
+        ex, s_ex, _ = run_self(a.config, 0)
+        gd, s_gd, _ = run_self(a.config, 2, W)
         rec, hit, tot = record_recall(ex, gd)
         print(json.dumps({"config": a.config, "record_recall": rec, "records_exact": tot, "records_found": hit,
-                          "scored": sum(s["cand_scored"] for s in s_gd),
-                          "skipped": sum(s["cand_skipped"] for s in s_gd),
-                          "aligned_exact": sum(s["naligned"] for s in s_ex),
-                          "aligned_gated": sum(s["naligned"] for s in s_gd)}))
+                          "pairs_scored": sum(s["cand_scored"] for s in s_gd),
+                          "pairs_dropped": sum(s["cand_skipped"] for s in s_gd),
+                          "report_s_exact": sum(s["t_report"] for s in s_ex),
+                          "report_s_gated": sum(s["t_report"] for s in s_gd),
+                          "total_s_exact": sum(s["t_total"] for s in s_ex),
+                          "total_s_gated": sum(s["t_total"] for s in s_gd)}))
         for p in (ex, gd):
             shutil.rmtree(os.path.dirname(os.path.dirname(p)), ignore_errors=True)
 

@@ -87,16 +87,58 @@ def mutate(template: np.ndarray, rng, p_ins: float, p_del: float, p_sub: float) 
     return out
 
 
+def homopolymer_mask(t: np.ndarray, min_run: int) -> np.ndarray:
+    """True at every position of a run of one base at least `min_run` long."""
+    n = t.shape[0]
+    if n == 0:
+        return np.zeros(0, dtype=bool)
+    edges = np.flatnonzero(np.diff(t)) + 1
+    starts = np.concatenate([[0], edges])
+    runs = np.diff(np.concatenate([starts, [n]]))
+    return np.repeat(runs >= min_run, runs)
+
+
+def mutate_hp(template: np.ndarray, rng, p_ins: float, p_del: float, p_sub: float, hp_bias: float,
+              hp_min: int = 4) -> np.ndarray:
+    """mutate() with insertions and deletions `hp_bias` times as likely inside
+    homopolymer runs of at least `hp_min` bases as elsewhere, scaled so the
+    read's expected indel rates stay p_ins and p_del."""
+    n = template.shape[0]
+    w = np.where(homopolymer_mask(template, hp_min), hp_bias, 1.0)
+    w /= w.mean() if n else 1.0
+    u = rng.random(n)
+    ins = rng.random(n) < np.minimum(p_ins * w, 1.0)
+    ins_base = rng.integers(0, 4, size=n, dtype=np.uint8)
+    sub_off = rng.integers(1, 4, size=n, dtype=np.uint8)
+    pd = np.minimum(p_del * w, 1.0)
+    keep = u >= pd
+    sub = keep & (u < pd + p_sub)
+    base = template.copy()
+    base[sub] = (base[sub] + sub_off[sub]) & 3
+    cnt = ins.astype(np.int64) + keep.astype(np.int64)
+    out = np.empty(int(cnt.sum()), dtype=np.uint8)
+    pos = np.cumsum(cnt) - cnt
+    out[pos[ins]] = ins_base[ins]
+    out[(pos + ins.astype(np.int64))[keep]] = base[keep]
+    return out
+
+
 def simulate_reads(genome: np.ndarray, coverage: float, mean_len: float, sd_len: float,
                    min_len: int, seed: int, p_ins=0.09, p_del=0.04, p_sub=0.02,
-                   max_len: int | None = None) -> ReadSet:
+                   max_len: int | None = None, hp_bias: float | None = None,
+                   blocks: tuple | None = None) -> ReadSet:
     """Reads at `coverage`x with lognormal template lengths (mean, sd), clipped
-    below at `min_len` by resampling and above at `max_len` (and the genome)."""
+    below at `min_len` by resampling and above at `max_len` (and the genome).
+    hp_bias: indels that many times as likely in homopolymer runs >= 4
+    (mutate_hp).  blocks = (size_mbp, n): stop once the reads fill n DBsplit
+    blocks of size_mbp (db/DBsplit.c:204-236) -- the first n blocks of the
+    full read set, read for read."""
     rng = np.random.Generator(np.random.PCG64(seed))
     mu, sigma = lognormal_params(mean_len, sd_len)
     G = genome.shape[0]
     target = coverage * G
     total = 0
+    btot, bdone = 0, 0
     reads, starts, lens, strands = [], [], [], []
     while total < target:
         L = int(rng.lognormal(mu, sigma))
@@ -104,7 +146,10 @@ def simulate_reads(genome: np.ndarray, coverage: float, mean_len: float, sd_len:
             continue
         s = int(rng.integers(0, G - L + 1))
         t = genome[s:s + L]
-        r = mutate(t, rng, p_ins, p_del, p_sub)
+        if hp_bias is None:
+            r = mutate(t, rng, p_ins, p_del, p_sub)
+        else:
+            r = mutate_hp(t, rng, p_ins, p_del, p_sub, hp_bias)
         st = int(rng.integers(0, 2))
         if st:
             r = (3 - r[::-1]).astype(np.uint8)
@@ -113,7 +158,33 @@ def simulate_reads(genome: np.ndarray, coverage: float, mean_len: float, sd_len:
         lens.append(L)
         strands.append(st)
         total += L
+        if blocks is not None:
+            btot += len(r)
+            if btot >= blocks[0] * 1000000:
+                btot = 0
+                bdone += 1
+                if bdone == blocks[1]:
+                    break
     return ReadSet(reads, np.array(starts), np.array(lens), np.array(strands))
+
+
+def repeat_model_genome(length: int, seed: int, families: int = 200, family_len: int = 6000, copies: int = 40,
+                        divergence: float = 0.10, tandem_frac: float = 0.01, periods=(2, 50),
+                        tandem_lens=(500, 5000)) -> np.ndarray:
+    """iid genome with `families` interspersed repeat families (a random
+    consensus of family_len bases, `copies` copies each at `divergence`
+    split 9:4:2 into insertions / deletions / substitutions, at uniform
+    positions) and tandem repeats (period uniform in `periods`, array length
+    uniform in `tandem_lens`) until they cover tandem_frac of the genome."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    tandems = []
+    covered = 0
+    while covered < tandem_frac * length:
+        per = int(rng.integers(periods[0], periods[1] + 1))
+        tl = int(rng.integers(tandem_lens[0], tandem_lens[1] + 1))
+        tandems.append((per, tl))
+        covered += tl
+    return repeat_genome(length, seed, families=[(family_len, copies, divergence)] * families, tandems=tandems)
 
 
 def write_fasta(path: str, reads, prefix: str = "sim", width: int = 80) -> None:
@@ -199,3 +270,28 @@ def config_5mbp(seed: int = 2):
     """5 Mbp iid genome, 30x, lognormal(10 kb, 6 kb, min 1 kb), 9/4/2 % errors."""
     g = random_genome(5_000_000, seed)
     return simulate_reads(g, 30.0, 10000.0, 6000.0, 1000, seed + 1000)
+
+
+# ---- the repeat-rich and long-read workloads --------------------------------------
+
+def config_repeats_50mbp(seed: int = 4, blocks: int = 1):
+    """50 Mbp iid genome with 200 repeat families (6 kb consensus, 40 copies at
+    10 % divergence) and 1 % tandem repeats (period 2-50); reads at 30x,
+    lognormal(10 kb, 6 kb, min 1 kb), 12 % error split 9:4:2.  Only the
+    first `blocks` DBsplit blocks of 200 Mbp are simulated (the full set's
+    first reads, read for read)."""
+    g = repeat_model_genome(50_000_000, seed)
+    e = 0.12
+    return simulate_reads(g, 30.0, 10000.0, 6000.0, 1000, seed + 1000, p_ins=e * 9 / 15, p_del=e * 4 / 15,
+                          p_sub=e * 2 / 15, blocks=(200, blocks))
+
+
+def config_nanopore_100mbp(seed: int = 5, blocks: int = 2, block_mbp: int = 400):
+    """100 Mbp iid genome with the same repeat model; reads at 30x,
+    lognormal(25 kb, 20 kb) up to 200 kb (min 1 kb), 8 % error split 9:4:2
+    with indels 3x as likely in homopolymer runs >= 4.  Only the first
+    `blocks` DBsplit blocks of block_mbp are simulated."""
+    g = repeat_model_genome(100_000_000, seed)
+    e = 0.08
+    return simulate_reads(g, 30.0, 25000.0, 20000.0, 1000, seed + 1000, p_ins=e * 9 / 15, p_del=e * 4 / 15,
+                          p_sub=e * 2 / 15, max_len=200_000, hp_bias=3.0, blocks=(block_mbp, blocks))

@@ -22,6 +22,10 @@ DATA = {
     "edges": lambda: sim.config_band_edges(21),
     "at_rich": lambda: sim.config_at_rich(31),
     "self_rep": lambda: sim.config_self_repeats(41),
+    # the first 200 Mbp block of the 50 Mbp repeat genome (200 families x 40 copies, 1 % tandems) at 30x, 12 %
+    "repeats_50mbp": lambda: sim.config_repeats_50mbp(4),
+    # the first two 400 Mbp blocks of the 100 Mbp genome at 30x, nanopore-like reads (mean 25 kb, 8 %)
+    "nanopore_100mbp": lambda: sim.config_nanopore_100mbp(5),
 }
 
 # name -> (data, DBsplit size in Mbp, daligner flags, block arguments)
@@ -76,8 +80,15 @@ CASES = {
     "c2_mdust": ("5mbp", 400, ["-mdust", "-j8", "-M32"], ["C2.1", "C2.1"]),
     # the same reads in ~40 Mbp blocks: block 2 against blocks 1-4 (symmetric: the .las of both sides)
     "c2_blocks": ("5mbp", 40, ["-j8", "-M32"], ["C2.2", "C2.1", "C2.2", "C2.3", "C2.4"]),
+    # a memory limit that caps k-mer groups below MAXGRAM (dalign/filter.c:2107-2174)
+    "c2_M4": ("5mbp", 400, ["-j8", "-M4"], ["C2.1", "C2.1"]),
+    # the repeat-rich workload of the learned scorer: one 200 Mbp block against itself
+    "c4_j8": ("repeats_50mbp", 200, ["-j8", "-M32"], ["R.1", "R.1"]),
+    # the nanopore-like block pair: 400 Mbp against 400 Mbp at -e.75
+    "c5_e75": ("nanopore_100mbp", 400, ["-e.75", "-j8", "-M32"], ["N.1", "N.2"]),
 }
-LARGE = {"c2_j8", "c2_mdust", "c2_blocks"}   # minutes on one CPU core: the oracle runs them only when MI_SLOW=1
+# minutes on one CPU core: the oracle runs them only when MI_SLOW=1
+LARGE = {"c2_j8", "c2_mdust", "c2_blocks", "c2_M4", "c4_j8", "c5_e75"}
 
 # -m tracks a case's DB carries: name -> (layout, source).  "dust" is the
 # reference DBdust output for the reads (a fixture); "rnd<seed>" random

@@ -199,3 +199,125 @@ def test_kmer_keys_that_do_not_fit_are_an_error(tmp_path):
     r = subprocess.run([EXE, "-k20", "-j4", "-M16", "C1.1", "C1.1"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 1 and "64-bit k-mer keys" in r.stderr, r.stderr[-1000:]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["rep_j8"])
+def test_redundancy_branches_fire(case, tmp_path, monkeypatch):
+    """Handle_Redundancies (dalign/filter.c:1403-1566) takes its fusion and
+    replacement branches on these goldens: the counters of the report
+    kernel's lane-0 restatement are non-zero for fusions of an earlier-
+    starting kept alignment with a later one reaching further (with and
+    without COMP, i.e. in both passes) and for containment / equal starts,
+    and the .las is still the reference's.  (The other order, a new
+    alignment starting before a kept one that reaches further, needs a
+    reverse wave running back past an earlier alignment's start; it did not
+    occur on these inputs.  On the iid 5 Mbp reads the function fires once
+    in a pass: repeats are what make alignments of one pair overlap.)"""
+    from marvel_amd.daligner import Daligner
+
+    monkeypatch.setenv("MI_REDUND_STATS", "1")
+    print(f"[redundancy branches] case {case}", flush=True)
+    _, _, flags, blocks = las_cases.CASES[case]
+    opts = {f[1]: f[2:] for f in flags}
+    las_cases.make_db(case, str(tmp_path))
+    dal = Daligner()
+    dal.configure(nthreads=int(opts["j"]), mem_limit_gb=int(opts["M"]), minover=2000)
+    dal.set_filter_params(14, 6, 0, 35)
+    A = dal.open_block(str(tmp_path / blocks[0]))
+    spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=int(opts["j"]))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        s1, s2 = dal.compare_self(A, spec, 1, blocks[0])
+    finally:
+        os.chdir(cwd)
+    assert _shas(str(tmp_path)) == GOLD[case]["las"]
+    br = [a + b for a, b in zip(s1["redund"], s2["redund"])]
+    print(f"[redundancy branches] {case}: {br}", flush=True)
+    assert br[6] + br[7] > 0, br   # k first, j reaches further: fused, A and B views
+    assert br[4] > 0 and br[5] > 0, br   # j contains k; same start, j longer
+    assert s1["redund"][7] > 0 and s2["redund"][6] > 0, (s1["redund"], s2["redund"])   # not COMP / COMP
+
+
+def _in_process(dal, case, d, tuning=None):
+    """One in-process self comparison of a golden case in directory d; returns (shas, stats)."""
+    _, _, flags, blocks = las_cases.CASES[case]
+    opts = {f[1]: f[2:] for f in flags}
+    nthreads = int(opts.get("j", "4"))
+    dal.configure(nthreads=nthreads, mem_limit_gb=int(opts["M"]), minover=2000)
+    dal.set_filter_params(14, 6, 0, 35)
+    A = dal.open_block(os.path.join(d, blocks[0]))
+    spec = dal.new_align_spec(0.70, 100, list(A.freq), nthreads=nthreads)
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        for f in os.listdir(d):
+            if f.startswith("001_"):
+                for g in os.listdir(os.path.join(d, f)):
+                    os.remove(os.path.join(d, f, g))
+        stats = dal.compare_self(A, spec, 1, blocks[0])
+    finally:
+        os.chdir(cwd)
+    return _shas(d), stats
+
+
+@pytest.mark.gpu
+def test_graph_replays_match_reference(tmp_path):
+    """The same block pair three times in one process, another block pair in
+    between: the first run records its radix sorts and scans as HIP graphs,
+    the later ones replay them between the program's own eager work and
+    synchronisations; every .las is the reference's."""
+    from marvel_amd.daligner import Daligner
+
+    print("[graphs] c1_j8 x3 with rep_j8 between", flush=True)
+    d1, d2 = str(tmp_path / "c1"), str(tmp_path / "rep")
+    las_cases.make_db("c1_j8", d1)
+    las_cases.make_db("rep_j8", d2)
+    dal = Daligner()
+    replayed = []
+    for i, case in enumerate(["c1_j8", "rep_j8", "c1_j8", "c1_j8"]):
+        shas, (s1, s2) = _in_process(dal, case, d1 if case == "c1_j8" else d2)
+        assert shas == GOLD[case]["las"], (i, case)
+        replayed.append(s1["graphs_replayed"] + s2["graphs_replayed"])
+        print(f"[graphs] run {i} {case}: captured {s1['graphs_captured'] + s2['graphs_captured']}, replayed "
+              f"{replayed[-1]} ({s1['launches_replayed'] + s2['launches_replayed']} launches)", flush=True)
+    assert replayed[2] > 0 and replayed[3] > 0
+
+
+@pytest.mark.gpu
+def test_tuning_keeps_output(tmp_path):
+    """Launch shapes other than the defaults (fewer waves, smaller pebble
+    pools: more re-runs at wider windows) leave the .las and the batched
+    aligner's results unchanged."""
+    import ctypes as C
+
+    import numpy as np
+
+    from marvel_amd.align import gen_tasks_device, la_batch_device
+    from marvel_amd.daligner import Daligner
+    from marvel_amd.lib import MiTuning, lib
+
+    print("[tuning] c1_j8 and rep_j8 with small grids and pools", flush=True)
+    dt = gen_tasks_device(3000, 7)
+    ref = la_batch_device(dt)
+    t = MiTuning(12, 13, 4, 8)
+    assert lib.mi_set_tuning(C.byref(t)) == 0
+    try:
+        dal = Daligner()
+        for case in ("c1_j8", "rep_j8"):
+            d = str(tmp_path / case)
+            las_cases.make_db(case, d)
+            shas, (s1, s2) = _in_process(dal, case, d)
+            assert shas == GOLD[case]["las"], case
+            print(f"[tuning] {case}: retries {s1['retries'] + s2['retries']}", flush=True)
+        got = la_batch_device(dt)
+    finally:
+        lib.mi_set_tuning(None)
+    a, b = ref.fields(), got.fields()
+    for k in ("status", "abpos", "aepos", "bbpos", "bepos", "diffs", "atlen", "btlen"):
+        assert np.array_equal(a[k], b[k]), k
+    ta, tb = ref.traces(), got.traces()
+    for i in range(dt.n):
+        assert np.array_equal(ta[a["atrace"][i]:a["atrace"][i] + a["atlen"][i]],
+                              tb[b["atrace"][i]:b["atrace"][i] + b["atlen"][i]]), i

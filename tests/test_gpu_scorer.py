@@ -1,7 +1,8 @@
 """Candidate scorer on the GPU: the MFMA MLP against a float32 reference,
-capture mode leaves the .las bit-exact, gating with an open threshold is the
-exact path, and the shipped weights keep >= 99.5 % of the records on data
-they were not trained on."""
+capture mode leaves the .las bit-exact and labels every candidate pair with
+the records its exact run wrote, gating with an open threshold is the exact
+path, and the shipped weights keep >= 99.5 % of the records on data they
+were not trained on."""
 
 import hashlib
 import json
@@ -10,8 +11,8 @@ import shutil
 
 import numpy as np
 import pytest
-import torch
 
+import scorer_ref as R
 from marvel_amd import scorer as S
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -31,25 +32,21 @@ def _cleanup(las):
     shutil.rmtree(os.path.dirname(os.path.dirname(las)), ignore_errors=True)
 
 
-def _model(seed):
-    torch.manual_seed(seed)
-    return torch.nn.Sequential(torch.nn.Linear(16, 64), torch.nn.ReLU(), torch.nn.Linear(64, 64), torch.nn.ReLU(),
-                               torch.nn.Linear(64, 1))
-
-
 @pytest.mark.gpu
 def test_mlp_kernel_matches_fp32_reference():
     from marvel_amd.daligner import Daligner
     print("[scorer] MFMA MLP vs float32 reference", flush=True)
     sc = S.GpuScorer(Daligner().h)
-    m = _model(7)
-    W = S.weights_from_torch(m)
-    sc.load(W)
+    p = R.init_params(7)
     rng = np.random.default_rng(8)
+    p[1024:1088] = rng.normal(0, 0.2, 64)
+    p[5184:5248] = rng.normal(0, 0.2, 64)
+    W = S.weights_from_vector(p)
+    sc.load(W)
     for n in (1, 31, 33, 1000, 4099):
         X = rng.uniform(0, 2, (n, 16)).astype(np.float32)
         got = sc.logits(X)
-        ref32 = m(torch.from_numpy(X)).detach().numpy()[:, 0]
+        ref32 = R.forward(X, p, dtype=np.float32)[3]
         refk = S.reference_logits(X, W)
         # against the reference that rounds to bf16 where the kernel does:
         # only the float32 summation order differs
@@ -62,12 +59,19 @@ def test_mlp_kernel_matches_fp32_reference():
 @pytest.mark.parametrize("config", ["small", "small_repeats"])
 def test_capture_keeps_exact_output(config):
     print(f"[scorer] capture {config}", flush=True)
-    las, stats, X, y = S.run_self(config, 1)
+    las, stats, (X, nrec, work) = S.run_self(config, 1)
     try:
         assert _sha(las) == _gold(config)
-        assert X.shape == (len(y), 16) and np.isfinite(X).all()
-        assert len(y) == sum(s["naligned"] for s in stats)
-        assert 0 < y.sum() <= len(y)
+        assert X.shape == (len(nrec), 16) and np.isfinite(X).all()
+        # one row per pair past the gate pre-check; their records are the .las records
+        assert len(nrec) == sum(s["njobs_kept"] for s in stats)
+        assert int(nrec.sum()) == len(S.las_keys(las))
+        assert (work[nrec > 0] > 0).all()
+        assert 0 < (nrec > 0).sum() < len(nrec)
+        # features in their ranges: shares in [0, 1], COMP 0 / 1 (forward then complement pass)
+        for j in (1, 13, 14, 15):
+            assert (X[:, j] >= 0).all() and (X[:, j] <= 1).all(), j
+        assert set(np.unique(X[:, 12])) == {0.0, 1.0}
     finally:
         _cleanup(las)
 
@@ -75,13 +79,13 @@ def test_capture_keeps_exact_output(config):
 @pytest.mark.gpu
 def test_gate_with_open_threshold_is_exact():
     print("[scorer] gate, threshold -inf", flush=True)
-    W = S.weights_from_torch(_model(9))
+    W = S.weights_from_vector(R.init_params(9))
     W.threshold = -3.0e38
-    las, stats, _, _ = S.run_self("small", 2, W)
+    las, stats, _ = S.run_self("small", 2, W)
     try:
         assert _sha(las) == _gold("small")
         assert sum(s["cand_skipped"] for s in stats) == 0
-        assert sum(s["cand_scored"] for s in stats) == sum(s["naligned"] for s in stats)
+        assert sum(s["cand_scored"] for s in stats) == sum(s["njobs_kept"] for s in stats)
     finally:
         _cleanup(las)
 
@@ -94,8 +98,8 @@ def test_gate_recall_held_out(config):
     if config in W.meta.get("configs", []):
         pytest.skip(f"{config} is training data")
     print(f"[scorer] gate recall {config}", flush=True)
-    ex, _, _, _ = S.run_self(config, 0)
-    gd, stats, _, _ = S.run_self(config, 2, W)
+    ex, _, _ = S.run_self(config, 0)
+    gd, stats, _ = S.run_self(config, 2, W)
     try:
         rec, hit, tot = S.record_recall(ex, gd)
         print(f"recall {rec:.5f} ({hit}/{tot}), skipped {sum(s['cand_skipped'] for s in stats)} of "

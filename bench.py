@@ -145,6 +145,16 @@ def gated_leg(steps):
         ref = os.path.join(d, "exact.las")
         shutil.copyfile(las, ref)
         sc = S.GpuScorer(dal.h)
+        # the most any pair-level pruning can save at full recall: the aligner
+        # work of the candidate pairs whose exact run writes no record
+        sc.mode(1)
+        try:
+            run_steps(dal, A, spec, dev, cA, d, "R.1", 1, 0)
+        finally:
+            sc.mode(0)
+        _, nrec, work = sc.take()
+        work = work.astype("float64")
+        share = float(work[nrec == 0].sum() / max(work.sum(), 1.0))
         sc.load(W)
         sc.mode(2)
         try:
@@ -167,6 +177,9 @@ def gated_leg(steps):
                           "pairs_scored": sum(x["cand_scored"] for x in s_gate),
                           "pairs_dropped": sum(x["cand_skipped"] for x in s_gate)},
                 "speedup": round(t_plain / t_gate, 4),
+                "aligner_work_without_records": round(share, 5),
+                "pruning_ceiling": round(1.0 / max(1e-9, 1.0 - share), 4),
+                "candidate_pairs": int(len(nrec)), "candidate_pairs_without_records": int((nrec == 0).sum()),
                 "record_recall": round(rec, 6), "records_exact": tot, "records_of_exact_kept": hit,
                 "threshold": W.threshold, "trained_on": W.meta.get("configs"), "setup_s": round(t_db, 2)}
     finally:
@@ -461,8 +474,8 @@ def main():
                                "note": "one step whose index build uploads the block's reads from host memory"},
             "roofline": {"bound": "valu", "kernel": "k_report (one wave64 per read pair: gating + Local_Alignment)",
                          "achieved": round(ach, 2), "peak": VALU_PEAK_GOPS, "unit": "Gop/s (W_align lane-ops)",
-                         "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": profiled_traffic("k_report<1, 0>"),
-                         "traffic_unit": "HBM bytes per k_report<1, 0> launch (rocprofv3 --pmc, calibrated)",
+                         "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": profiled_traffic("k_report<1, false>"),
+                         "traffic_unit": "HBM bytes per k_report<1, false> launch (rocprofv3 --pmc, calibrated)",
                          "kernel_ms_per_step": round(k_report * 1e3, 2), "launches_per_step": launches,
                          "w_align_per_step": w_align},
             "roofline_hbm": {"bound": "hbm", "kernel": "seed LSD radix sort (rs_hist + rs_scatter)",

@@ -270,14 +270,16 @@ int mi_device_memset(void *p, int value, long long bytes);
  * not depend on them): waves per CU of the report kernel's 64-diagonal
  * window (4..32, default 32) and of its 128-diagonal window (2..32, default
  * 16), the pebble pool of the 64-diagonal window as a power of two (12..18,
- * default 15), waves per CU of the batched aligner (4..32, default 32).
- * A 0 field keeps the default; NULL restores all defaults.  The tuner in
- * marvel_amd/tune.py picks them from a calibration run. */
+ * default 15), waves per CU of the batched aligner's 64-diagonal window
+ * (4..32, default 32) and of its 256- and 1024-diagonal windows (2..20,
+ * default 8).  A 0 field keeps the default; NULL restores all defaults.
+ * The tuner in marvel_amd/tune.py picks them from a calibration run. */
 typedef struct {
     int report_waves_per_cu;
     int report_pool_log2;
     int report_l1_waves_per_cu;
     int batch_waves_per_cu;
+    int batch_wide_waves_per_cu;
 } MiTuning;
 int mi_set_tuning(const MiTuning *t);
 int mi_get_tuning(MiTuning *t);

@@ -497,7 +497,8 @@ __global__ void k_prefilter(const int* __restrict__ jobs, int njobs, const long 
 
 // ---- candidate scorer features (no counterpart in the reference) ------------
 // A candidate is a read pair that passed the gate pre-check (a job of the
-// report kernel).  Its seeds are sorted by apos; one thread streams them into
+// report kernel).  Its seeds are sorted by apos; one thread streams them (at
+// most kFeatSeeds of them, evenly spaced, for pairs with more) into
 // a table of up to kBinSlots diagonal bins (band = diag >> binshift, the
 // report_thread bands), each with its seed count, first and last apos, the
 // sum and sum of squares of the apos spacing of consecutive seeds, and its
@@ -506,7 +507,7 @@ __global__ void k_prefilter(const int* __restrict__ jobs, int njobs, const long 
 // one with the fewest seeds.  The 16 features describe the strongest bin
 // (most seeds), the runner-up and the pair:
 //   0 log2(1 + seeds) / 16            8 log2(1 + alen) / 16
-//   1 seeds in the best bin / seeds   9 log2(1 + blen) / 16
+//   1 best bin / seeds examined       9 log2(1 + blen) / 16
 //   2 log2(1 + best bin seeds) / 16  10 log2(1 + overlap the best bin's diagonal implies) / 16
 //   3 log2(1 + runner-up seeds) / 16 11 best bin apos span / that overlap (capped at 2)
 //   4 best bin apos span / alen      12 COMP
@@ -515,6 +516,7 @@ __global__ void k_prefilter(const int* __restrict__ jobs, int njobs, const long 
 //   7 log2(1 + spacing sd) / 16      15 short-period k-mers / seeds
 constexpr int kBinSlots = 8;
 constexpr int kJobFeat = 16;
+constexpr int kFeatSeeds = 256;   // seeds examined per pair at most (evenly spaced)
 
 __device__ __forceinline__ bool short_period_kmer(const uint64_t* A, int apos, int K)
 {
@@ -560,7 +562,11 @@ __global__ __launch_bounds__(128) void k_job_features(const int* __restrict__ jo
     const int alen = lenA[ar], blen = lenB[br];
     const uint64_t* A = pkA + w0A[ar];
     int lc_all = 0;
-    for (long long i = s; i < e; i++) {
+    // pairs with many seeds (repeats) are sampled evenly: the pass stays short
+    const long long step = (e - s > kFeatSeeds) ? (e - s + kFeatSeeds - 1) / kFeatSeeds : 1;
+    int nsamp = 0;
+    for (long long i = s; i < e; i += step) {
+        nsamp++;
         const Seed x = h[i];
         const int bin = x.diag >> binshift;
         const int lc = short_period_kmer(A, x.apos, K) ? 1 : 0;
@@ -590,7 +596,7 @@ __global__ __launch_bounds__(128) void k_job_features(const int* __restrict__ jo
         if (T[q][1] > T[b1][1]) { b2 = b1; b1 = q; }
         else if (b2 < 0 || T[q][1] > T[b2][1]) b2 = q;
     }
-    const float n = (float)(e - s);
+    const float n = (float)nsamp;
     const float c = (float)T[b1][1];
     const float span = (float)(T[b1][3] - T[b1][2]);
     const float mean = c > 1.f ? F[b1][0] / (c - 1.f) : 0.f;
@@ -598,7 +604,7 @@ __global__ __launch_bounds__(128) void k_job_features(const int* __restrict__ jo
     const int dg = T[b1][0] * (1 << binshift) + (1 << binshift) / 2;
     const int ov = max(min(alen, blen + dg) - max(0, dg), 0);
     auto lg = [](float v) { return __log2f(1.f + v) * (1.f / 16.f); };
-    out[0] = lg(n);
+    out[0] = lg((float)(e - s));
     out[1] = c / n;
     out[2] = lg(c);
     out[3] = lg(b2 >= 0 ? (float)T[b2][1] : 0.f);

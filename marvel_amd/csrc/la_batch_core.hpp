@@ -240,9 +240,10 @@ static BatchPlan make_plan(int maxlen, int tspace, int cus)
     p.pts_cap = maxlen / tspace + 64;
     p.latr_cap = 4 * (2 * (maxlen / tspace + 2));
     const int w0 = mi_tuning().batch_waves_per_cu;
+    const int w1 = mi_tuning().batch_wide_waves_per_cu;
     p.lv[0] = Level{1, 1 << 15, cus * (w0 ? w0 : 32)};   // default 8 waves per SIMD
-    p.lv[1] = Level{4, 1 << 17, cus * 8};
-    p.lv[2] = Level{16, 1 << 17, cus * 8};
+    p.lv[1] = Level{4, 1 << 17, cus * (w1 ? w1 : 8)};
+    p.lv[2] = Level{16, 1 << 17, cus * (w1 ? w1 : 8)};
     p.lv[3] = Level{16, 1 << 20, 256};
     return p;
 }

@@ -192,13 +192,15 @@ int mi_device_memset(void* p, int value, long long bytes)
 int mi_set_tuning(const MiTuning* t)
 {
     if (t == nullptr) {
-        mi::g_tuning = MiTuning{0, 0, 0, 0};
+        mi::g_tuning = MiTuning{0, 0, 0, 0, 0};
         return 0;
     }
     const bool ok = (t->report_waves_per_cu == 0 || (t->report_waves_per_cu >= 4 && t->report_waves_per_cu <= 32)) &&
                     (t->report_pool_log2 == 0 || (t->report_pool_log2 >= 12 && t->report_pool_log2 <= 18)) &&
                     (t->report_l1_waves_per_cu == 0 || (t->report_l1_waves_per_cu >= 2 && t->report_l1_waves_per_cu <= 32)) &&
-                    (t->batch_waves_per_cu == 0 || (t->batch_waves_per_cu >= 4 && t->batch_waves_per_cu <= 32));
+                    (t->batch_waves_per_cu == 0 || (t->batch_waves_per_cu >= 4 && t->batch_waves_per_cu <= 32)) &&
+                    (t->batch_wide_waves_per_cu == 0 ||
+                     (t->batch_wide_waves_per_cu >= 2 && t->batch_wide_waves_per_cu <= 20));
     if (!ok) {
         mi::mi_set_error("mi_set_tuning: value out of range");
         return MI_ERR_ARG;

@@ -38,7 +38,7 @@ class MiGenTaskParams(C.Structure):
 
 class MiTuning(C.Structure):
     _fields_ = [("report_waves_per_cu", C.c_int), ("report_pool_log2", C.c_int), ("report_l1_waves_per_cu", C.c_int),
-                ("batch_waves_per_cu", C.c_int)]
+                ("batch_waves_per_cu", C.c_int), ("batch_wide_waves_per_cu", C.c_int)]
 
 
 class MiTraceTask(C.Structure):

@@ -47,7 +47,7 @@ NFEAT, NHID = 16, 64
 NPARAM = NFEAT * NHID + NHID + NHID * NHID + NHID + NHID + 1   # w1, b1, w2, b2, w3, b3
 FEATURES = [
     "log2(1 + seeds of the pair) / 16",
-    "seeds in the strongest diagonal bin / seeds",
+    "seeds in the strongest diagonal bin / seeds examined (at most 256, evenly spaced)",
     "log2(1 + seeds in the strongest bin) / 16",
     "log2(1 + seeds in the runner-up bin) / 16",
     "apos span of the strongest bin / alen",
@@ -61,7 +61,7 @@ FEATURES = [
     "COMP",
     "short-period k-mers in the strongest bin / its seeds",
     "bins with >= minhit seeds / 8 (8 bins tracked)",
-    "short-period k-mers / seeds",
+    "short-period k-mers / seeds examined",
 ]
 
 
@@ -291,6 +291,8 @@ def _reads(config: str):
         return sim.config_5mbp(2).reads
     if config == "repeats_50mbp":   # the first 200 Mbp block of the 50 Mbp repeat genome at 30x, 12 % error
         return sim.config_repeats_50mbp(4).reads
+    if config == "repeats_train":   # the same repeat model on another 20 Mbp genome (training data)
+        return sim.config_repeats_train(14).reads
     raise ValueError(f"unknown config {config}")
 
 
@@ -324,7 +326,7 @@ def run_self(config: str, mode: int, W: ScorerWeights | None = None, workdir: st
     from . import dbio
     from .daligner import Daligner
     d = workdir or tempfile.mkdtemp(prefix="miscorer_", dir="/dev/shm" if os.access("/dev/shm", os.W_OK) else None)
-    big = config in ("5mbp", "repeats_50mbp")
+    big = config in ("5mbp", "repeats_50mbp", "repeats_train")
     dbio.write_db(os.path.join(d, "S"), _reads(config), size_mbp=400)
     dal = Daligner()
     sc = GpuScorer(dal.h)

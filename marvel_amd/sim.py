@@ -295,3 +295,15 @@ def config_nanopore_100mbp(seed: int = 5, blocks: int = 2, block_mbp: int = 400)
     e = 0.08
     return simulate_reads(g, 30.0, 25000.0, 20000.0, 1000, seed + 1000, p_ins=e * 9 / 15, p_del=e * 4 / 15,
                           p_sub=e * 2 / 15, max_len=200_000, hp_bias=3.0, blocks=(block_mbp, blocks))
+
+
+def config_repeats_train(seed: int = 14):
+    """Training data for the learned scorer with the repeat model of
+    config_repeats_50mbp on another, smaller genome: 20 Mbp iid with 80
+    repeat families (6 kb consensus, 40 copies at 10 %) and 1 % tandems,
+    reads at 30x, lognormal(10 kb, 6 kb, min 1 kb), 12 % error; the first
+    100 Mbp block."""
+    g = repeat_model_genome(20_000_000, seed, families=80)
+    e = 0.12
+    return simulate_reads(g, 30.0, 10000.0, 6000.0, 1000, seed + 1000, p_ins=e * 9 / 15, p_del=e * 4 / 15,
+                          p_sub=e * 2 / 15, blocks=(100, 1))

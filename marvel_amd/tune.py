@@ -2,8 +2,8 @@
 
 The report kernel's waves per CU at the 64- and 128-diagonal windows, the
 pebble pool of the 64-diagonal window and the batched aligner's waves per CU
-(MiTuning, include/marvl_align.h) change how fast a pass runs, never what it
-writes.  The tuner times short calibration runs of the caller's own
+at its 64-diagonal and its wider windows (MiTuning, include/marvl_align.h)
+change how fast a pass runs, never what it writes.  The tuner times short calibration runs of the caller's own
 workload with each candidate, keeps the fastest, and stores the choice per
 device (architecture and compute units) in a JSON cache, so later processes
 on the same kind of GPU load it instead of calibrating again.
@@ -20,11 +20,15 @@ import time
 
 from .lib import MiTuning, lib
 
-FIELDS = ("report_waves_per_cu", "report_pool_log2", "report_l1_waves_per_cu", "batch_waves_per_cu")
-DEFAULT = {"report_waves_per_cu": 32, "report_pool_log2": 15, "report_l1_waves_per_cu": 16, "batch_waves_per_cu": 32}
+FIELDS = ("report_waves_per_cu", "report_pool_log2", "report_l1_waves_per_cu", "batch_waves_per_cu",
+          "batch_wide_waves_per_cu")
+DEFAULT = {"report_waves_per_cu": 32, "report_pool_log2": 15, "report_l1_waves_per_cu": 16, "batch_waves_per_cu": 32,
+           "batch_wide_waves_per_cu": 8}
 REPORT_CANDIDATES = [(w, p) for w in (16, 24, 32) for p in (14, 15, 16)]
 L1_CANDIDATES = (8, 16, 32)
-BATCH_CANDIDATES = (16, 24, 32)
+# (64-diagonal window, wider windows): the levels are timed apart, so each
+# run scores one candidate of each and the two are chosen independently
+BATCH_CANDIDATES = ((16, 8), (24, 12), (32, 16), (32, 20))
 
 
 def cache_path() -> str:
@@ -106,15 +110,20 @@ def tune_report(step) -> dict:
 
 def tune_batch(run) -> dict:
     """Picks the batched aligner's waves per CU with `run()` (one batch
-    returning its work list: kernel microseconds per window level at [2:6])."""
+    returning its work list: kernel microseconds per window level at [2:6]).
+    The 64-diagonal level's time depends only on its own waves and the wider
+    levels' only on theirs (the tasks each level re-runs do not depend on the
+    launch shape), so one run per candidate pair scores both."""
     base = current()
-    timings = {}
-    for w in BATCH_CANDIDATES:
-        apply({**base, "batch_waves_per_cu": w})
+    narrow, wide = {}, {}
+    for w0, w1 in BATCH_CANDIDATES:
+        apply({**base, "batch_waves_per_cu": w0, "batch_wide_waves_per_cu": w1})
         work = run()
-        timings[f"b{w}"] = sum(work[2:6]) / 1e6
-    best = {**base, "batch_waves_per_cu": min(BATCH_CANDIDATES, key=lambda c: timings[f"b{c}"])}
+        narrow[w0] = min(narrow.get(w0, float("inf")), work[2] / 1e6)
+        wide[w1] = sum(work[3:6]) / 1e6
+    best = {**base, "batch_waves_per_cu": min(narrow, key=narrow.get), "batch_wide_waves_per_cu": min(wide, key=wide.get)}
     apply(best)
+    timings = {**{f"b{k}": v for k, v in narrow.items()}, **{f"wide{k}": v for k, v in wide.items()}}
     return {"choice": best, "timings_s": {k: round(v, 5) for k, v in timings.items()}}
 
 

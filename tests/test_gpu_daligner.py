@@ -301,7 +301,7 @@ def test_tuning_keeps_output(tmp_path):
     print("[tuning] c1_j8 and rep_j8 with small grids and pools", flush=True)
     dt = gen_tasks_device(3000, 7)
     ref = la_batch_device(dt)
-    t = MiTuning(12, 13, 4, 8)
+    t = MiTuning(12, 13, 4, 8, 3)
     assert lib.mi_set_tuning(C.byref(t)) == 0
     try:
         dal = Daligner()

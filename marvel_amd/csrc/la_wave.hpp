@@ -14,10 +14,11 @@
 //
 // Pebbles (trace-point cells, dalign/align.c:359-365) are appended to a
 // per-wave pool in HBM; lanes that cross a trace-point boundary in the same
-// wave allocate with a ballot + mbcnt prefix.  The mark and chain depth of
-// every diagonal's current pebbles are carried in registers, so the
-// reference's `cells[ha].mark` test needs no memory read and the trace is
-// rebuilt with a single walk per chain.
+// wave allocate with a ballot + mbcnt prefix.  A diagonal carries only the
+// index of its current A and B pebble: the reference's `cells[ha].mark` test
+// reads the pebble's mark back from the pool in the (rare) waves where a
+// lane reaches a trace point, and the trace is rebuilt by one walk per chain
+// that also counts its length.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -214,15 +215,14 @@ __device__ __forceinline__ int prefix_rank(unsigned long long m)
 struct Diag {
     int V, M, NA, NB;
     uint64_t T;
-    int HA, HAm, HAd;   // A pebble: index, mark, chain depth
-    int HB, HBm, HBd;   // B pebble
+    int HA, HB;   // current A / B pebble (index in the pool)
 };
 
 __device__ __forceinline__ Diag diag_zero(int sent)
 {
     Diag d;
     d.V = sent; d.M = 0; d.NA = 0; d.NB = 0; d.T = 0;
-    d.HA = d.HAm = d.HAd = d.HB = d.HBm = d.HBd = 0;
+    d.HA = d.HB = 0;
     return d;
 }
 
@@ -231,8 +231,7 @@ __device__ __forceinline__ Diag diag_shfl(const Diag& D, int sl)
     Diag g;
     g.V = shfl_i(D.V, sl); g.M = shfl_i(D.M, sl); g.NA = shfl_i(D.NA, sl); g.NB = shfl_i(D.NB, sl);
     g.T = shfl_u64(D.T, sl);
-    g.HA = shfl_i(D.HA, sl); g.HAm = shfl_i(D.HAm, sl); g.HAd = shfl_i(D.HAd, sl);
-    g.HB = shfl_i(D.HB, sl); g.HBm = shfl_i(D.HBm, sl); g.HBd = shfl_i(D.HBd, sl);
+    g.HA = shfl_i(D.HA, sl); g.HB = shfl_i(D.HB, sl);
     return g;
 }
 
@@ -247,8 +246,7 @@ __device__ __forceinline__ Diag diag_pick(const Diag& D, int sel)
     Diag g;
     g.V = pick3(D.V, sel); g.M = pick3(D.M, sel); g.NA = pick3(D.NA, sel); g.NB = pick3(D.NB, sel);
     g.T = ((uint64_t)(uint32_t)pick3((int)(uint32_t)(D.T >> 32), sel) << 32) | (uint32_t)pick3((int)(uint32_t)D.T, sel);
-    g.HA = pick3(D.HA, sel); g.HAm = pick3(D.HAm, sel); g.HAd = pick3(D.HAd, sel);
-    g.HB = pick3(D.HB, sel); g.HBm = pick3(D.HBm, sel); g.HBd = pick3(D.HBd, sel);
+    g.HA = pick3(D.HA, sel); g.HB = pick3(D.HB, sel);
     return g;
 }
 
@@ -257,8 +255,7 @@ __device__ __forceinline__ Diag diag_rdl(const Diag& D, int ln)
     Diag g;
     g.V = rdl(D.V, ln); g.M = rdl(D.M, ln); g.NA = rdl(D.NA, ln); g.NB = rdl(D.NB, ln);
     g.T = rdl64(D.T, ln);
-    g.HA = rdl(D.HA, ln); g.HAm = rdl(D.HAm, ln); g.HAd = rdl(D.HAd, ln);
-    g.HB = rdl(D.HB, ln); g.HBm = rdl(D.HBm, ln); g.HBd = rdl(D.HBd, ln);
+    g.HA = rdl(D.HA, ln); g.HB = rdl(D.HB, ln);
     return g;
 }
 
@@ -269,8 +266,8 @@ struct WaveCount {
 
 struct PassOut {
     int trimx, trimy, trimd;
-    int tha, thad, thb, thbd;   // tail pebbles of the A and B chains, and their depths
-    int used;                   // pebbles allocated by the pass
+    int tha, thb;   // tail pebbles of the A and B chains
+    int used;       // pebbles allocated by the pass
     int status;
 };
 
@@ -312,8 +309,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
     // state of the pass (names follow dalign/align.c:386-401)
     int besta = mida, besty = (mida - hgh) >> 1;
     int trima = mida, trimy = besty, trimd = 0;
-    int trimha = 0, trimhad = 0, trimhb = 1, trimhbd = 0;   // pebbles 0/1 = roots of the first visited diagonal
-    int morea = mida, morey = besty, mored = 0, moreha = 0, morehad = 0, morehb = 1, morehbd = 0;
+    int trimha = 0, trimhb = 1;   // pebbles 0/1 = roots of the first visited diagonal
+    int morea = mida, morey = besty, mored = 0, moreha = 0, morehb = 1;
     int morem = -1, lasta = mida, more = 1;
     int dif = 0;
     int my_snake = 0;   // this lane's snake compare steps in the pass
@@ -363,8 +360,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             const int ha = 2 * idx, hb = ha + 1;
             pool[ha] = Peb{-1, k, 0, hamark};
             pool[hb] = Peb{-1, k, 0, hbmark};
-            P[j].HA = ha; P[j].HAm = hamark; P[j].HAd = 0;
-            P[j].HB = hb; P[j].HBm = hbmark; P[j].HBd = 0;
+            P[j].HA = ha;
+            P[j].HB = hb;
             bd[j].NA = na;
             bd[j].NB = nb;
             const int x = y + k;
@@ -411,13 +408,19 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 any |= needA[j] | needB[j];
             }
             if (!__any(any)) break;
+            wave_sync();   // the pebbles pushed so far are visible to the mark reads below
 #pragma unroll
             for (int j = 0; j < NCH; j++) {
                 const int k = base + 64 * j + lane;
                 Diag& D = bd[j];
                 Diag& G = P[j];
-                const bool pushA = needA[j] && (dif == 0 || ((DIR > 0) ? (G.HAm < D.NA) : (G.HAm > D.NA)));
-                const bool pushB = needB[j] && (dif == 0 || ((DIR > 0) ? (G.HBm < D.NB) : (G.HBm > D.NB)));
+                // marks of the current pebbles (a pebble pushed in an earlier
+                // round of this wave is read back by the lane that wrote it)
+                int gam = 0, gbm = 0;
+                if (needA[j] && dif != 0) gam = pool[G.HA].mark;
+                if (needB[j] && dif != 0) gbm = pool[G.HB].mark;
+                const bool pushA = needA[j] && (dif == 0 || ((DIR > 0) ? (gam < D.NA) : (gam > D.NA)));
+                const bool pushB = needB[j] && (dif == 0 || ((DIR > 0) ? (gbm < D.NB) : (gbm > D.NB)));
                 const unsigned long long ba = __ballot(pushA), bb = __ballot(pushB);
                 const int na_ = __popcll(ba), nb_ = __popcll(bb);
                 if (navail + na_ + nb_ > pool_cap) {
@@ -427,13 +430,13 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 if (pushA) {
                     const int idx = navail + prefix_rank(ba);
                     pool[idx] = Peb{G.HA, k, dif, D.NA};
-                    G.HA = idx; G.HAm = D.NA; G.HAd += 1;
+                    G.HA = idx;
                 }
                 navail += na_;
                 if (pushB) {
                     const int idx = navail + prefix_rank(bb);
                     pool[idx] = Peb{G.HB, k, dif, D.NB};
-                    G.HB = idx; G.HBm = D.NB; G.HBd += 1;
+                    G.HB = idx;
                 }
                 navail += nb_;
                 if (needA[j]) D.NA += DIR * TS;
@@ -501,8 +504,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                     if (j == jj) {
                         trima = rdl(cN[j], ln);
                         trimy = rdl(yN[j], ln);
-                        trimha = rdl(P[j].HA, ln); trimhad = rdl(P[j].HAd, ln);
-                        trimhb = rdl(P[j].HB, ln); trimhbd = rdl(P[j].HBd, ln);
+                        trimha = rdl(P[j].HA, ln);
+                        trimhb = rdl(P[j].HB, ln);
                     }
                 trimd = dif;
             }
@@ -519,8 +522,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             Diag& D = bd[j];
             D.V = (k >= low && k <= hgh) ? cN[j] : SENT;
             D.M = mN[j]; D.T = bN[j];
-            D.HA = P[j].HA; D.HAm = P[j].HAm; D.HAd = P[j].HAd;
-            D.HB = P[j].HB; D.HBm = P[j].HBm; D.HBd = P[j].HBd;
+            D.HA = P[j].HA; D.HB = P[j].HB;
         }
         wc.cells += hgh - low + 1;
         wc.steps += 1;
@@ -563,7 +565,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                     const Diag g = fetch_diag<NCH>(bd, base, kk);
                     if (morem <= g.M) {
                         morem = g.M; morea = g.V; morey = (morea - kk) / 2; mored = dif;
-                        moreha = g.HA; morehad = g.HAd; morehb = g.HB; morehbd = g.HBd;
+                        moreha = g.HA; morehb = g.HB;
                     }
                 }
             }
@@ -746,14 +748,14 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         out.trimx = morea - morey;
         out.trimy = morey;
         out.trimd = mored;
-        out.tha = moreha; out.thad = morehad;
-        out.thb = morehb; out.thbd = morehbd;
+        out.tha = moreha;
+        out.thb = morehb;
     } else {
         out.trimx = trima - trimy;
         out.trimy = trimy;
         out.trimd = trimd;
-        out.tha = trimha; out.thad = trimhad;
-        out.thb = trimhb; out.thbd = trimhbd;
+        out.tha = trimha;
+        out.thb = trimhb;
     }
     out.used = navail;
     long long t = my_snake;
@@ -766,27 +768,35 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
 // ---- trace assembly ----------------------------------------------------------
 // Lane 0 rebuilds the A-view chain and lane 1 the B-view chain of one pass
 // (the same code runs on both lanes with per-lane coordinates).  The chain
-// from the tail pebble back to its root is copied into pts[0..depth] in
-// root-to-tail order, then walked like dalign/align.c:970-1107 (forward) or
-// :1709-1901 (reverse).
-//   tr     : trace origin (forward writes tr[0..], reverse prepends tr[-1..])
+// from the tail pebble back to its root is copied tail first to the end of
+// pts[0..pts_cap), so it reads root to tail from pts + pts_cap - (depth + 1),
+// then walked like dalign/align.c:970-1107 (forward) or :1709-1901 (reverse).
+//   tr     : trace origin (forward writes tr[0..], reverse prepends tr[-1..]);
+//            it holds maxtp values on either side
 //   fwdlen : (reverse only) length of the forward part already at tr[0..]
-// Returns the new trace length / first index through *len / *first.
+// Returns the new trace length / first index through *len / *first, or
+// false for a chain that leaves the pool, outgrows pts_cap or needs more
+// than maxtp trace values.
 template <int DIR>
-__device__ bool assemble_chain(const Peb* pool, int pool_used, Peb* pts, int tail, int depth, bool isB, int mida,
-                               int TS, int boff, int trimx, int trimy, int trimd, uint16_t* tr, int fwdlen,
+__device__ bool assemble_chain(const Peb* pool, int pool_used, Peb* pts_all, int pts_cap, int tail, bool isB, int mida,
+                               int TS, int boff, int trimx, int trimy, int trimd, uint16_t* tr, int maxtp, int fwdlen,
                                int* len, int* first, int* root_diag)
 {
-    // copy the chain, tail first; a chain that does not end at a root after
-    // `depth` links is reported, never followed out of the pool
-    int h = tail;
-    for (int d = depth; d >= 0; d--) {
-        if (h < 0 || h >= pool_used) return false;
+    // copy the chain, tail first, until its root (up == -1); a chain that
+    // leaves the pool or does not end within pts_cap links is reported,
+    // never followed further
+    int h = tail, nl = 0;
+    while (h != -1) {
+        if (h < 0 || h >= pool_used || nl >= pts_cap) return false;
         const Peb p = pool[h];
-        pts[d] = p;
+        pts_all[pts_cap - 1 - nl] = p;
         h = p.up;
+        nl++;
     }
-    if (h != -1) return false;
+    if (nl == 0) return false;
+    const int depth = nl - 1;
+    if (2 * depth + 2 > maxtp) return false;
+    const Peb* pts = pts_all + (pts_cap - nl);
     const int tprim = isB ? trimy : trimx;   // end-point coordinate along the trace axis
     const int tsec = isB ? trimx : trimy;    // and across it
     // secondary coordinate of a pebble: A-view -> y, B-view -> x
@@ -895,16 +905,13 @@ __device__ LaResult local_alignment(const DevSpec& sp, const uint64_t* A, int al
     PassOut f = wave_pass<NCH, +1>(sp, A, alen, B, blen, comp, low, hgh, anti, minp, maxp, pool, pool_cap, abort_flag,
                                    wc);
     if (f.status != LA_OK) { r.status = f.status; return r; }
-    // each trace holds at most maxtp values on either side of its origin
-    if (max(f.thad, f.thbd) + 1 > pts_cap || 2 * max(f.thad, f.thbd) + 2 > maxtp) { r.status = LA_CHAIN; return r; }
     wave_sync();
     int len = 0, first = 0, rootk = 0;
     bool ok = true;
     if (lane < 2) {
         const bool isB = (lane == 1);
-        ok = assemble_chain<+1>(pool, f.used, pts + (isB ? pts_cap : 0), isB ? f.thb : f.tha, isB ? f.thbd : f.thad,
-                                isB, anti, TS, boff, f.trimx, f.trimy, f.trimd, isB ? btr : atr, 0, &len, &first,
-                                &rootk);
+        ok = assemble_chain<+1>(pool, f.used, pts + (isB ? pts_cap : 0), pts_cap, isB ? f.thb : f.tha, isB, anti, TS,
+                                boff, f.trimx, f.trimy, f.trimd, isB ? btr : atr, maxtp, 0, &len, &first, &rootk);
     }
     if (__any(!ok)) { r.status = LA_CHAIN; return r; }
     const int fa = rdl(len, 0), fb = rdl(len, 1);
@@ -919,15 +926,14 @@ __device__ LaResult local_alignment(const DevSpec& sp, const uint64_t* A, int al
     PassOut v = wave_pass<NCH, -1>(sp, A, alen, B, blen, comp, mind, mind, anti, minp, maxp, pool, pool_cap,
                                    abort_flag, wc);
     if (v.status != LA_OK) { r.status = v.status; return r; }
-    if (max(v.thad, v.thbd) + 1 > pts_cap || 2 * max(v.thad, v.thbd) + 2 > maxtp) { r.status = LA_CHAIN; return r; }
     wave_sync();
     len = 0;
     first = 0;
     if (lane < 2) {
         const bool isB = (lane == 1);
-        ok = assemble_chain<-1>(pool, v.used, pts + (isB ? pts_cap : 0), isB ? v.thb : v.tha, isB ? v.thbd : v.thad,
-                                isB, anti, TS, boff, v.trimx, v.trimy, v.trimd, isB ? btr : atr, isB ? fb : fa, &len,
-                                &first, &rootk);
+        ok = assemble_chain<-1>(pool, v.used, pts + (isB ? pts_cap : 0), pts_cap, isB ? v.thb : v.tha, isB, anti, TS,
+                                boff, v.trimx, v.trimy, v.trimd, isB ? btr : atr, maxtp, isB ? fb : fa, &len, &first,
+                                &rootk);
     }
     if (__any(!ok)) { r.status = LA_CHAIN; return r; }
     const int ra = rdl(len, 0), rb = rdl(len, 1);

@@ -36,7 +36,10 @@ int main(int argc, char** argv)
     if (match < .2) bias = 3;
     int ave_path = (int)(60 * (1. - kBias[bias] * (1. - corr)));
     int mscore = (int)(1000 * kBias[bias] * (1. - corr)), dscore = 1000 - mscore;
-    std::vector<int16_t> table(32768), score(32768);
+    // the device reads the score table right after the suffix table (one base pointer)
+    std::vector<int16_t> tables(65536);
+    int16_t* table = tables.data();
+    int16_t* score = tables.data() + 32768;
     for (int x = 0; x < 32768; x++) {
         int run = 0, top = 0;
         for (int i = 14; i >= 0; i--) {
@@ -46,7 +49,7 @@ int main(int argc, char** argv)
         table[x] = (int16_t)(run - top);
         score[x] = (int16_t)run;
     }
-    mi::DevSpec sp{table.data(), score.data(), ts, ave_path};
+    mi::DevSpec sp{table, score, ts, ave_path};
     emu::Ctx ctx;
     emu::g = &ctx;
     for (int i = 0; i < n; i++) {

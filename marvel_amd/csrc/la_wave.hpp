@@ -319,6 +319,11 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
     uint64_t bN[NCH];
     bool stA[NCH], stB[NCH];
     Diag P[NCH];   // per diagonal: the predecessor's fields, updated by this wave's pebbles
+    // Chunk j holds diagonals [base + 64 j, base + 64 j + 63].  A chunk wholly
+    // outside the band [low, hgh] holds only sentinels (V = SENT, no record,
+    // no clip, no pebble), so the per-chunk steps skip it: a wave-uniform
+    // test, always true for the 64-diagonal window.
+    auto live = [&](int j) { return NCH == 1 || (base + 64 * j + 63 >= low && base + 64 * j <= hgh); };
 
     // ---- wave 0 (dalign/align.c:455-587, :1193-1322) --------------------
     {
@@ -400,6 +405,8 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             bool needA[NCH], needB[NCH], any = false;
 #pragma unroll
             for (int j = 0; j < NCH; j++) {
+                needA[j] = needB[j] = false;
+                if (!live(j)) continue;
                 const int k = base + 64 * j + lane;
                 const bool in = (k >= low && k <= hgh);
                 const int y = yN[j];
@@ -411,6 +418,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             wave_sync();   // the pebbles pushed so far are visible to the mark reads below
 #pragma unroll
             for (int j = 0; j < NCH; j++) {
+                if (!live(j)) continue;
                 const int k = base + 64 * j + lane;
                 Diag& D = bd[j];
                 Diag& G = P[j];
@@ -454,6 +462,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
 #pragma unroll
             for (int jj = 0; jj < NCH; jj++) {
                 const int j = (DIR > 0) ? (NCH - 1 - jj) : jj;
+                if (!live(j)) continue;
                 const int c = cN[j];
                 const int incl = (DIR > 0) ? wave_suffix_max(c) : wave_prefix_min(c);
                 const int excl = (DIR > 0) ? lane_above_or(incl, SENT) : lane_below_or(incl, SENT);
@@ -518,6 +527,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         // fields.
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
+            if (!live(j)) continue;   // its V is SENT since the last wave's sentinel step
             const int k = base + 64 * j + lane;
             Diag& D = bd[j];
             D.V = (k >= low && k <= hgh) ? cN[j] : SENT;
@@ -533,6 +543,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             int bclip_k = (DIR > 0) ? -INT32_MAX : INT32_MAX;
 #pragma unroll
             for (int j = 0; j < NCH; j++) {
+                if (!live(j)) continue;
                 const unsigned long long ba = __ballot(stA[j]), bb = __ballot(stB[j]);
                 if (ba) {
                     more = 0;   // forward keeps the last visited = lowest k; reverse the highest
@@ -577,6 +588,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
             int nlo = INT32_MAX, nhi = -INT32_MAX;
 #pragma unroll
             for (int j = 0; j < NCH; j++) {
+                if (!live(j)) continue;
                 const int k = base + 64 * j + lane;
                 const bool keep = (k >= low && k <= hgh) && ((DIR > 0) ? (bd[j].V >= n) : (bd[j].V <= n));
                 const unsigned long long bk = __ballot(keep);
@@ -590,6 +602,10 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         }
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
+            if (!live(j)) {
+                bd[j].V = SENT;
+                continue;
+            }
             const int k = base + 64 * j + lane;
             if (!(k >= low && k <= hgh)) bd[j].V = SENT;
         }
@@ -645,6 +661,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         int Vm[NCH], Vp[NCH];
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
+            if (!live(j)) continue;
             int up = lane_below(bd[j].V);
             int dn = lane_above(bd[j].V);
             const int upc = (j > 0) ? rdl(bd[j - 1].V, 63) : SENT;       // wave-uniform reads
@@ -657,6 +674,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         // new end diagonals take NA/NB of their inner neighbour
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
+            if (!live(j)) continue;   // the chunks holding low and hgh are live
             const int k = base + 64 * j + lane;
             int naU = lane_below(bd[j].NA), nbU = lane_below(bd[j].NB);
             int naD = lane_above(bd[j].NA), nbD = lane_above(bd[j].NB);
@@ -676,6 +694,11 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         bool undef = false;
 #pragma unroll
         for (int j = 0; j < NCH; j++) {
+            if (!live(j)) {
+                cN[j] = SENT;
+                stA[j] = stB[j] = false;
+                continue;
+            }
             const int k = base + 64 * j + lane;
             const bool in = (k >= low && k <= hgh);
             const int am = Vm[j], ac = bd[j].V, ap = Vp[j];

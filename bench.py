@@ -284,7 +284,9 @@ def c5_leg(with_cpu=True):
                     os.symlink(os.path.join(d, f), os.path.join(sd, f))
             print(f"[c5] reference daligner {' '.join(cf)} on {ncpu} CPUs", flush=True)
             t = time.time()
-            subprocess.run([refexe] + cf + ["N.1", "N.2"], cwd=sd, check=True, capture_output=True, timeout=1000)
+            # -v: the reference reports each index build and comparison as it goes (to stderr here)
+            subprocess.run([refexe, "-v"] + cf + ["N.1", "N.2"], cwd=sd, check=True, stdout=sys.stderr,
+                           stderr=sys.stderr, timeout=1000)
             dt = time.time() - t
             n = struct.unpack("<q", open(os.path.join(sd, "001_00001/N.1.N.2.las"), "rb").read(8))[0]
             out["cpu"] = {"value": round(2 * n / dt, 1), "unit": "overlaps/s", "s": round(dt, 2), "cores": j,

@@ -208,6 +208,10 @@ static int launch_level(int nch, int grid, const MiLaTask* tasks, const int* ord
         hipLaunchKernelGGL(HIP_KERNEL_NAME(la_batch_kernel<1>), dim3(grid), dim3(64), 0, st, tasks, order, n, pk, sp,
                            res, arena, arena_cap, arena_top, retry, nretry, ws, counter, work, abort_flag);
         break;
+    case 2:
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(la_batch_kernel<2>), dim3(grid), dim3(64), 0, st, tasks, order, n, pk, sp,
+                           res, arena, arena_cap, arena_top, retry, nretry, ws, counter, work, abort_flag);
+        break;
     case 4:
         hipLaunchKernelGGL(HIP_KERNEL_NAME(la_batch_kernel<4>), dim3(grid), dim3(64), 0, st, tasks, order, n, pk, sp,
                            res, arena, arena_cap, arena_top, retry, nretry, ws, counter, work, abort_flag);
@@ -241,9 +245,11 @@ static BatchPlan make_plan(int maxlen, int tspace, int cus)
     p.latr_cap = 4 * (2 * (maxlen / tspace + 2));
     const int w0 = mi_tuning().batch_waves_per_cu;
     const int w1 = mi_tuning().batch_wide_waves_per_cu;
+    // windows of 64, 128, 256 and 1,024 diagonals; a task re-runs at the
+    // next level when its band outgrows the window or its pool fills
     p.lv[0] = Level{1, 1 << 15, cus * (w0 ? w0 : 32)};   // default 8 waves per SIMD
-    p.lv[1] = Level{4, 1 << 17, cus * (w1 ? w1 : 8)};
-    p.lv[2] = Level{16, 1 << 17, cus * (w1 ? w1 : 8)};
+    p.lv[1] = Level{2, 1 << 17, cus * (w1 ? w1 : 8)};
+    p.lv[2] = Level{4, 1 << 17, cus * (w1 ? w1 : 8)};
     p.lv[3] = Level{16, 1 << 20, 256};
     return p;
 }

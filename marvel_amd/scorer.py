@@ -297,21 +297,28 @@ def _reads(config: str):
 
 
 def train_gpu(sc: GpuScorer, X: np.ndarray, nrec: np.ndarray, work: np.ndarray, recall: float = 0.999,
-              epochs: int = 40, batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> ScorerWeights:
+              epochs: int = 40, batch: int = 4096, lr: float = 2e-3, seed: int = 0,
+              groups: np.ndarray | None = None) -> ScorerWeights:
     """Trains the MLP on the GPU from captured rows (sample_weights) and sets
-    the threshold that keeps `recall` of the training records."""
+    the threshold that keeps `recall` of the training records of every group
+    (the training configs: the lowest of the per-group thresholds, so a small
+    config is not outweighed by a large one)."""
     w = sample_weights(nrec, work)
     y = (np.asarray(nrec) > 0).astype(np.float32)
     p, losses = sc.train(X, y, w, init_params(seed), epochs=epochs, batch=batch, lr=lr, seed=seed)
     W = weights_from_vector(p)
     sc.load(W)
     lg = sc.logits(X)
-    W.threshold = choose_threshold(lg, nrec, recall)
+    nrec = np.asarray(nrec)
+    groups = np.zeros(len(nrec), dtype=np.int64) if groups is None else np.asarray(groups)
+    W.threshold = min(choose_threshold(lg[groups == g], nrec[groups == g], recall) for g in np.unique(groups))
     keep = lg >= W.threshold
     work = np.asarray(work, dtype=np.float64)
     W.meta = {"epochs": epochs, "seed": seed, "n": int(len(y)), "pos": int(y.sum()), "recall_target": recall,
               "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
               "train_record_recall": float(np.asarray(nrec)[keep].sum() / max(1, np.asarray(nrec).sum())),
+              "train_record_recall_per_group": [float(nrec[keep & (groups == g)].sum() /
+                                                      max(1, nrec[groups == g].sum())) for g in np.unique(groups)],
               "train_pairs_dropped": float(1 - keep.mean()),
               "train_work_dropped": float(work[~keep].sum() / max(1.0, work.sum()))}
     return W
@@ -358,7 +365,8 @@ def main(argv=None):
     t = sub.add_parser("train", help="train on captured .npz files (GPU)")
     t.add_argument("inputs", nargs="+")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
-    t.add_argument("--recall", type=float, default=0.999, help="share of the training records the threshold keeps")
+    t.add_argument("--recall", type=float, default=0.999,
+                   help="share of each input's training records the threshold keeps")
     t.add_argument("--epochs", type=int, default=40)
     e = sub.add_parser("evaluate", help="record recall and time of gating (GPU)")
     e.add_argument("--config", default="repeats_50mbp")
@@ -372,16 +380,17 @@ def main(argv=None):
         shutil.rmtree(os.path.dirname(os.path.dirname(las)), ignore_errors=True)
     elif a.cmd == "train":
         from .daligner import Daligner
-        Xs, rs, ws, cfgs = [], [], [], []
-        for p in a.inputs:
+        Xs, rs, ws, gs, cfgs = [], [], [], [], []
+        for i, p in enumerate(a.inputs):
             z = np.load(p)
             Xs.append(z["X"])
             rs.append(z["nrec"])
             ws.append(z["work"])
+            gs.append(np.full(len(z["nrec"]), i))
             cfgs.append(str(z["config"]))
         sc = GpuScorer(Daligner().h)
         W = train_gpu(sc, np.concatenate(Xs), np.concatenate(rs), np.concatenate(ws), recall=a.recall,
-                      epochs=a.epochs)
+                      epochs=a.epochs, groups=np.concatenate(gs))
         W.meta["configs"] = cfgs
         W.save(a.out)
         print(json.dumps(W.meta))

@@ -108,7 +108,7 @@ __global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __rest
     // bases rpos-K+1 .. rpos, first base most significant
     const int b0 = rpos - K + 1;
     const int wi = b0 >> 5, sh = 2 * (b0 & 31);
-    // 2K <= 28 bits span at most two words
+    // the 2K <= 62 bits of a k-mer (K <= 31) span at most two words
     uint64_t win = w[wi] >> sh;
     if (sh) win |= w[wi + 1] << (64 - sh);
     uint64_t code = 0;

@@ -63,6 +63,8 @@ CASES = {
     "c1_b": ("small", 400, ["-b", "-j8", "-M16"], ["C1.1", "C1.1"]),
     "at_j8": ("at_rich", 400, ["-j8", "-M16"], ["C1.1", "C1.1"]),
     "at_b": ("at_rich", 400, ["-b", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "at_b_k16": ("at_rich", 400, ["-b", "-k16", "-j8", "-M16"], ["C1.1", "C1.1"]),
+    "at_b_k19": ("at_rich", 400, ["-b", "-k19", "-h50", "-j4", "-M16"], ["C1.1", "C1.1"]),
     "at_b_mrnd": ("at_rich", 400, ["-b", "-mrnd", "-j4", "-M16"], ["C1.1", "C1.1"]),
     "at_b_blocks": ("at_rich", 1, ["-b", "-j8", "-M16"], ["C1.3", "C1.1", "C1.2"]),
     # identity mode on reads holding copies of their own segments (direct and inverted): each read against

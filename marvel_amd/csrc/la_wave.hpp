@@ -185,8 +185,14 @@ __device__ __forceinline__ int prefix_max_nn(int x)
 // (DPP wave_shl:1 / wave_shr:1)
 __device__ __forceinline__ int lane_above_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xF, 0xF, false); }
 __device__ __forceinline__ int lane_below_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xF, 0xF, false); }
-// inclusive max over lanes >= L (x >= -1)
-__device__ __forceinline__ int wave_suffix_max(int x) { return suffix_max_nn(x + 1) - 1; }
+// inclusive max over lanes >= L (x >= -1): the lanes reversed by one
+// ds_bpermute each way around the DPP prefix scan (the LDS pipe is idle here,
+// the VALU is not)
+__device__ __forceinline__ int wave_suffix_max(int x)
+{
+    const int r = 63 - __lane_id();
+    return __shfl(prefix_max_nn(__shfl(x + 1, r, 64)), r, 64) - 1;
+}
 // inclusive min over lanes <= L (any x): the max of INT32_MAX - x >= 0
 __device__ __forceinline__ int wave_prefix_min(int x) { return 2147483647 - prefix_max_nn(2147483647 - x); }
 
@@ -241,12 +247,17 @@ __device__ __forceinline__ int pick3(int v, int sel)
     const int b = lane_below(v), a = lane_above(v);
     return sel < 0 ? b : (sel > 0 ? a : v);
 }
+// The predecessor's fields the pass carries on (M, T, HA, HB): one
+// ds_bpermute per field from lane (L + sel) & 63, on the LDS pipe rather
+// than as DPP moves and selects on the VALU.  V, NA and NB stay the lane's own.
 __device__ __forceinline__ Diag diag_pick(const Diag& D, int sel)
 {
+    const int src = (__lane_id() + sel) & 63;
     Diag g;
-    g.V = pick3(D.V, sel); g.M = pick3(D.M, sel); g.NA = pick3(D.NA, sel); g.NB = pick3(D.NB, sel);
-    g.T = ((uint64_t)(uint32_t)pick3((int)(uint32_t)(D.T >> 32), sel) << 32) | (uint32_t)pick3((int)(uint32_t)D.T, sel);
-    g.HA = pick3(D.HA, sel); g.HB = pick3(D.HB, sel);
+    g.V = D.V; g.NA = D.NA; g.NB = D.NB;
+    g.M = shfl_i(D.M, src);
+    g.T = shfl_u64(D.T, src);
+    g.HA = shfl_i(D.HA, src); g.HB = shfl_i(D.HB, src);
     return g;
 }
 
@@ -468,19 +479,23 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 const int excl = (DIR > 0) ? lane_above_or(incl, SENT) : lane_below_or(incl, SENT);
                 const int prior = (DIR > 0) ? max(excl, carry) : min(excl, carry);
                 const bool rec = (DIR > 0) ? (c > prior) : (c < prior);
-                bool okm, oktr = false;
-                if (dif == 0) {
-                    okm = rec;   // wave 0 records set besta, lasta and the trim point
-                    oktr = rec;
-                } else {
-                    okm = rec && (mN[j] >= sp.ave_path);
-                    if (okm) {
-                        const uint64_t b = bN[j];
-                        if (sp.table[b & kTrimMask] >= 0)
-                            oktr = (sp.table[(b >> kTrimLen) & kTrimMask] + sp.table[32768 + (b & kTrimMask)] >= 0);
+                // masks straight from the compares; the trim tables are read by
+                // the lanes of bm only
+                const unsigned long long brec = __ballot(rec);
+                unsigned long long bm = brec, bt = brec;   // wave 0 records set besta, lasta and the trim point
+                if (dif != 0) {
+                    bm = brec & __ballot(mN[j] >= sp.ave_path);
+                    bt = 0;
+                    if (bm) {
+                        int t = 0;
+                        if (rec && mN[j] >= sp.ave_path) {
+                            const uint64_t b = bN[j];
+                            if (sp.table[b & kTrimMask] >= 0)
+                                t = (sp.table[(b >> kTrimLen) & kTrimMask] + sp.table[32768 + (b & kTrimMask)] >= 0);
+                        }
+                        bt = __ballot(t != 0);
                     }
                 }
-                const unsigned long long brec = __ballot(rec), bm = __ballot(okm), bt = __ballot(oktr);
                 if (brec) {
                     const int ln = (DIR > 0) ? lowest_lane(brec) : highest_lane(brec);
                     best_k = base + 64 * j + ln;
@@ -676,6 +691,14 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         for (int j = 0; j < NCH; j++) {
             if (!live(j)) continue;   // the chunks holding low and hgh are live
             const int k = base + 64 * j + lane;
+            if constexpr (NCH == 1) {
+                // the new end lanes read their inner neighbour (one lane up /
+                // down, always inside the window), every other lane itself
+                const int src = lane + ((elo && k == low) ? 1 : 0) - ((ehi && k == hgh) ? 1 : 0);
+                bd[j].NA = shfl_i(bd[j].NA, src);
+                bd[j].NB = shfl_i(bd[j].NB, src);
+                continue;
+            }
             int naU = lane_below(bd[j].NA), nbU = lane_below(bd[j].NB);
             int naD = lane_above(bd[j].NA), nbD = lane_above(bd[j].NB);
             if (j > 0) {

@@ -1006,7 +1006,8 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     DBuf<Seed> s1, s2;
     s1.ensure(nh + 1);
     s2.ensure(nh + 1);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emit), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, B->keys.p, B->fmt, cnt.p, off.p, s1.p);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emit), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p,
+                       bs.p, B->keys.p, B->fmt, off.p, s1.p);
     HC(hipGetLastError());
     const int pa = A->fmt.pbits, ra = A->fmt.rbits, rb = B->fmt.rbits;
     if (pa + ra + rb > 64) throw Fail("blocks too large for 64-bit seed keys", MI_ERR_ARG);

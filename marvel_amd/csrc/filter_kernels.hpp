@@ -85,23 +85,49 @@ __global__ void k_revcomp(const uint64_t* __restrict__ in, uint64_t* __restrict_
     }
 }
 
-// one thread per k-mer: index -> (read, rpos) -> key.  The k-mers come in
-// runs of consecutive positions: kofs[s] = k-mers before run s; run s is
-// read seg_read[s] from last-base position seg_pos0[s] on (without a mask,
-// seg_read == nullptr: run r is all of read r from K-1 on).  rpos is the
-// position of the k-mer's last base (dalign/filter.c:400-490).
-__global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __restrict__ w0,
-                        const long long* __restrict__ kofs, int nrun, const int* __restrict__ seg_read,
-                        const int* __restrict__ seg_pos0, long long nk, int K, KeyFmt fmt, uint64_t* __restrict__ keys)
+// Order of the 2-bit groups of a word reversed (group 0 becomes group 31):
+// pairs swapped within nibbles, nibbles within bytes, then the bytes.
+__device__ __forceinline__ uint64_t reverse_bases(uint64_t x)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nk) return;
-    int lo = 0, hi = nrun;   // last run s with kofs[s] <= i
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    return __builtin_bswap64(x);
+}
+
+// last run s in [lo, hi) with kofs[s] <= i (kofs[lo] <= i holds)
+__device__ __forceinline__ int run_of(const long long* __restrict__ kofs, int lo, int hi, long long i)
+{
     while (hi - lo > 1) {
         const int m = (lo + hi) >> 1;
         if (kofs[m] <= i) lo = m;
         else hi = m;
     }
+    return lo;
+}
+
+// one thread per k-mer: index -> (read, rpos) -> key.  The k-mers come in
+// runs of consecutive positions: kofs[s] = k-mers before run s; run s is
+// read seg_read[s] from last-base position seg_pos0[s] on (without a mask,
+// seg_read == nullptr: run r is all of read r from K-1 on).  rpos is the
+// position of the k-mer's last base (dalign/filter.c:400-490).  Thread 0
+// finds the runs of the block's first and last k-mer; the other threads
+// search only between them (one run, mostly).
+__global__ __launch_bounds__(256) void k_kmers(const uint64_t* __restrict__ pk, const long long* __restrict__ w0,
+                                               const long long* __restrict__ kofs, int nrun,
+                                               const int* __restrict__ seg_read, const int* __restrict__ seg_pos0,
+                                               long long nk, int K, KeyFmt fmt, uint64_t* __restrict__ keys)
+{
+    __shared__ int span[2];
+    const long long i0 = (long long)blockIdx.x * blockDim.x;
+    if (threadIdx.x == 0) {
+        const long long i1 = min(i0 + (long long)blockDim.x, nk) - 1;
+        span[0] = run_of(kofs, 0, nrun, i0);
+        span[1] = run_of(kofs, span[0], nrun, i1);
+    }
+    __syncthreads();
+    const long long i = i0 + threadIdx.x;
+    if (i >= nk) return;
+    const int lo = run_of(kofs, span[0], span[1] + 1, i);
     const int r = seg_read ? seg_read[lo] : lo;
     const int rpos = (int)(i - kofs[lo]) + (seg_pos0 ? seg_pos0[lo] : K - 1);
     const uint64_t* w = pk + w0[r];
@@ -111,9 +137,8 @@ __global__ void k_kmers(const uint64_t* __restrict__ pk, const long long* __rest
     // the 2K <= 62 bits of a k-mer (K <= 31) span at most two words
     uint64_t win = w[wi] >> sh;
     if (sh) win |= w[wi + 1] << (64 - sh);
-    uint64_t code = 0;
-#pragma unroll 4
-    for (int j = 0; j < K; j++) code = (code << 2) | ((win >> (2 * j)) & 3);
+    // base j of the window at bits 2j: reversed, base 0 is the top group
+    const uint64_t code = reverse_bases(win) >> (64 - 2 * K);
     keys[i] = (code << fmt.cshift()) | ((uint64_t)r << fmt.pbits) | (uint64_t)rpos;
 }
 
@@ -334,24 +359,50 @@ __global__ void k_apply_limit(const long long* __restrict__ gidx, const unsigned
     if (gtot[gidx[i]] >= limit) cnt[i] = 0;
 }
 
-// seeds in merge order (dalign/filter.c:1210-1222): for every A entry its
-// B partners in index order
-__global__ void k_emit(const uint64_t* __restrict__ ak, KeyFmt fa, long long na, const long long* __restrict__ gidx,
-                       const long long* __restrict__ bs, const uint64_t* __restrict__ bk, KeyFmt fb,
-                       const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ off,
-                       Seed* __restrict__ out)
+// seed emission (merge_thread, dalign/filter.c:1061-1231): seeds in merge
+// order, for every A entry its B partners in index order.  A wave takes 64
+// consecutive A entries; each lane loads one entry (its seed range in the
+// exclusive scan `off`, na + 1 entries, its A position and read, the start
+// of its B partners) into the wave's LDS slots, then the wave writes the
+// entries' seeds in order, consecutive lanes to consecutive slots: lane q
+// finds the entry of seed q by bisecting the 64 ranges in LDS.
+__global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ ak, KeyFmt fa, long long na,
+                                              const long long* __restrict__ gidx, const long long* __restrict__ bs,
+                                              const uint64_t* __restrict__ bk, KeyFmt fb,
+                                              const unsigned long long* __restrict__ off, Seed* __restrict__ out)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= na) return;
-    const unsigned long long c = cnt[i];
-    if (c == 0) return;
-    const uint64_t a = ak[i];
-    const int ar = fa.read(a), ap = fa.pos(a);
-    const long long s = bs[gidx[i]];
-    Seed* o = out + off[i];
-    for (unsigned long long j = 0; j < c; j++) {
-        const uint64_t b = bk[s + j];
-        o[j] = Seed{ap - fb.pos(b), ap, ar, fb.read(b)};
+    __shared__ unsigned long long so[4][65];   // seed range starts of the wave's entries (and the end)
+    __shared__ long long sb[4][64];            // first B partner
+    __shared__ int sap[4][64], sar[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long e0 = ((long long)blockIdx.x * 4 + w) * 64;
+    if (e0 >= na) return;   // wave-uniform
+    const long long e = e0 + lane;
+    const int n = (int)min(64ll, na - e0);
+    if (lane < n) {
+        const uint64_t a = ak[e];
+        const unsigned long long o = off[e], o1 = off[e + 1];
+        so[w][lane] = o;
+        sb[w][lane] = (o1 > o) ? bs[gidx[e]] : 0;
+        sap[w][lane] = fa.pos(a);
+        sar[w][lane] = fa.read(a);
+        if (lane == n - 1) so[w][n] = o1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const unsigned long long first = so[w][0], total = so[w][n] - first;
+    for (unsigned long long q = lane; q < total; q += 64) {
+        const unsigned long long g = first + q;
+        int lo = 0, hi = n;   // last entry with so <= g
+        while (hi - lo > 1) {
+            const int m = (lo + hi) >> 1;
+            if (so[w][m] <= g) lo = m;
+            else hi = m;
+        }
+        const uint64_t b = bk[sb[w][lo] + (long long)(g - so[w][lo])];
+        const int ap = sap[w][lo];
+        out[g] = Seed{ap - fb.pos(b), ap, sar[w][lo], fb.read(b)};
     }
 }
 

@@ -80,6 +80,18 @@ inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
 {
     return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
 }
+inline int atomicMax(int* p, int v)
+{
+    int o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (o < v && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return o;
+}
+inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v)
+{
+    return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
+}
 
 // ---- runtime API --------------------------------------------------------------------
 
@@ -93,6 +105,7 @@ constexpr unsigned hipHostMallocMapped = 2;
 
 struct hipDeviceProp_t {
     int multiProcessorCount;
+    char gcnArchName[256];
 };
 
 inline const char* hipGetErrorString(hipError_t) { return "cpu emulation error"; }
@@ -158,6 +171,19 @@ inline hipError_t hipGetDeviceCount(int* n)
     *n = 1;
     return hipSuccess;
 }
+// HIP graphs: launches run as they are made here, so there is nothing to
+// record; the library's graph cache is switched off (MI_GRAPHS=0, set by
+// emu_runtime.cpp before main) and these only have to compile.
+typedef struct EmuGraph* hipGraph_t;
+typedef struct EmuGraphExec* hipGraphExec_t;
+constexpr hipError_t hipErrorNotSupported = 801;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return hipErrorNotSupported; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) { return hipErrorNotSupported; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
 typedef struct EmuEvent* hipEvent_t;
 inline hipError_t hipEventCreate(hipEvent_t* e)
 {
@@ -175,5 +201,6 @@ inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
 {
     const char* e = getenv("MI_EMU_CUS");
     p->multiProcessorCount = e ? atoi(e) : 2;
+    strcpy(p->gcnArchName, "cpu-emulation");
     return hipSuccess;
 }

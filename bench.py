@@ -436,6 +436,19 @@ def main():
         t_sort = sum(a["t_seed_sort"] + b["t_seed_sort"] for a, b in stats) / len(stats)
         sort_bytes = sum(x["nhits"] * 32 * x["seed_sort_passes"] for x in stats[-1])
         sort_gbs = sort_bytes / t_sort / 1e9 if t_sort > 0 else 0.0
+        # HBM bytes of one seed-sort pass from the committed counter summary:
+        # the digit-histogram read plus the scatter's reads and writes
+        hist_b, scat_b = profiled_traffic("rs_hist<Seed>"), profiled_traffic("rs_scatter<Seed>")
+        sort_traffic = hist_b + scat_b if hist_b and scat_b else None
+        # the filter stage (both strands): k-mer index builds, merge-join and
+        # seed sort, against implementation-independent bytes: per k-mer
+        # indexed its 8-byte key written and sorted once (read + write), per
+        # index entry merged 8 bytes read, per seed 16 bytes written and
+        # sorted once (read + write)
+        last = stats[-1]
+        filt_bytes = sum(24 * x["nkmers_b"] + 8 * (x["nkmers_a"] + x["nkmers_b"]) + 48 * x["nhits"] for x in last)
+        t_filt = phases[-1]["index"] + phases[-1]["complement_index"] + sum(x["t_filter"] for x in last)
+        filt_gbs = filt_bytes / t_filt / 1e9 if t_filt > 0 else 0.0
         # one step with the block uploaded from host memory (PCIe included)
         cwd = os.getcwd()
         os.chdir(d)
@@ -482,8 +495,16 @@ def main():
                          "w_align_per_step": w_align},
             "roofline_hbm": {"bound": "hbm", "kernel": "seed LSD radix sort (rs_hist + rs_scatter)",
                              "achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                             "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "traffic": sort_traffic,
+                             "traffic_unit": "HBM bytes per seed-sort pass, rs_hist + rs_scatter (rocprofv3 --pmc, "
+                                             "calibrated); algorithmic: 32 B per seed",
                              "ms_per_step": round(t_sort * 1e3, 2)},
+            "roofline_filter": {"bound": "hbm", "stage": "k-mer index builds, merge-join, seed sort; both strands",
+                                "achieved": round(filt_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(filt_gbs / HBM_PEAK_GBS, 4), "bytes_per_step": int(filt_bytes),
+                                "ms_per_step": round(t_filt * 1e3, 2),
+                                "bytes_model": "24 B per k-mer indexed, 8 B per index entry merged, "
+                                               "48 B per seed (written, then sorted once)"},
             "cpu_baseline": cpu,
             "tuning": tuning,
             "graphs": {"replayed_per_step": stats[-1][0]["graphs_replayed"] + stats[-1][1]["graphs_replayed"],

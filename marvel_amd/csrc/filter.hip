@@ -952,18 +952,32 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     const long long na = A->nk, nb = B->nk, ng = A->ng;
 
     // merge-join counts (count_thread, dalign/filter.c:948-1056)
+    // B range of every A group: with B the A index itself, the group's own
+    // entries (gstart[g], gstart[g + 1]) -- no copy
     DBuf<long long> bs, be;
     DBuf<u64> cnt, off, gtot, hist;
-    bs.ensure(ng);
-    be.ensure(ng);
+    if (!same_index) {
+        bs.ensure(ng);
+        be.ensure(ng);
+    }
+    const long long* bsp = same_index ? A->gstart.p : bs.p;
+    const long long* bep = same_index ? A->gstart.p + 1 : be.p;
     cnt.ensure(na);
     off.ensure(na + 1);
     gtot.ensure(ng);
     const int MAXGRAM = 10000;
     hist.ensure(MAXGRAM);
     HC(hipMemsetAsync(hist.p, 0, sizeof(u64) * MAXGRAM, st));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_brange), dim3(nblk(ng)), dim3(256), 0, st, A->keys.p, A->fmt, A->gstart.p, ng, B->keys.p, B->fmt, nb, same_index ? 1 : 0, bs.p, be.p);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bs.p, be.p, B->keys.p, B->fmt,
+    DBuf<long long> bound;
+    if (!same_index) {
+        const int nblock = (int)nblk(ng);
+        bound.ensure(2 * (size_t)nblock);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_brange_bounds), dim3(nblk(nblock)), dim3(256), 0, st, A->keys.p, A->fmt,
+                           A->gstart.p, ng, B->keys.p, B->fmt, nb, nblock, bound.p);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_brange), dim3(nblk(ng)), dim3(256), 0, st, A->keys.p, A->fmt,
+                           A->gstart.p, ng, B->keys.p, B->fmt, nb, bound.p, bs.p, be.p);
+    }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_counts), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p, bsp, bep, B->keys.p, B->fmt,
                        self ? (IDENTITY ? (comp ? 3 : 2) : 1) : 0, cnt.p);
     g_scan.run(cnt.p, off.p, na);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_totals), dim3(nblk(ng, 256 * 16)), dim3(256), 0, st, A->gstart.p, ng, off.p, gtot.p, hist.p, MAXGRAM);
@@ -1007,7 +1021,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     s1.ensure(nh + 1);
     s2.ensure(nh + 1);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emit), dim3(nblk(na)), dim3(256), 0, st, A->keys.p, A->fmt, na, A->gidx.p,
-                       bs.p, B->keys.p, B->fmt, off.p, s1.p);
+                       bsp, B->keys.p, B->fmt, off.p, s1.p);
     HC(hipGetLastError());
     const int pa = A->fmt.pbits, ra = A->fmt.rbits, rb = B->fmt.rbits;
     if (pa + ra + rb > 64) throw Fail("blocks too large for 64-bit seed keys", MI_ERR_ARG);

@@ -258,23 +258,37 @@ __device__ __forceinline__ long long lower_code(const uint64_t* keys, long long 
     return lo;
 }
 
-// per A group: the B range with the same code (find_tuple, dalign/filter.c:911-930).
-// With B the same index as A the range is the group itself.  Otherwise the
-// start is binary-searched and the end found by galloping from it: most codes
-// have a handful of entries, so the end is in the start's cache line.
-__global__ void k_group_brange(const uint64_t* __restrict__ ak, KeyFmt fa, const long long* __restrict__ gstart,
-                               long long ng, const uint64_t* __restrict__ bk, KeyFmt fb, long long nb, int same,
-                               long long* __restrict__ bs, long long* __restrict__ be)
+// per A group: the B range with the same code (find_tuple, dalign/filter.c:911-930),
+// for B another index than A (with B = A the range is the group itself and
+// needs no kernel).  The groups of a 256-thread block have ascending codes, so their starts lie in
+// the B range [first code's start, start of the code after the last one),
+// which k_brange_bounds finds for every block (one thread per block, all
+// blocks' searches in flight at once); each group then bisects only inside
+// its block's range and gallops to its end from its start (most codes have a
+// handful of entries, so the end is in the start's cache line).
+__global__ void k_brange_bounds(const uint64_t* __restrict__ ak, KeyFmt fa, const long long* __restrict__ gstart,
+                                long long ng, const uint64_t* __restrict__ bk, KeyFmt fb, long long nb, int nblock,
+                                long long* __restrict__ bound)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblock) return;
+    const long long g0 = (long long)t * 256, g1 = min(g0 + 256, ng) - 1;
+    const long long lo = lower_code(bk, nb, fa.code(ak[gstart[g0]]), fb);
+    bound[2 * t] = lo;
+    bound[2 * t + 1] = lo + lower_code(bk + lo, nb - lo, fa.code(ak[gstart[g1]]) + 1, fb);
+}
+
+__global__ __launch_bounds__(256) void k_group_brange(const uint64_t* __restrict__ ak, KeyFmt fa,
+                                                      const long long* __restrict__ gstart, long long ng,
+                                                      const uint64_t* __restrict__ bk, KeyFmt fb, long long nb,
+                                                      const long long* __restrict__ bound,
+                                                      long long* __restrict__ bs, long long* __restrict__ be)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
-    if (same) {
-        bs[g] = gstart[g];
-        be[g] = gstart[g + 1];
-        return;
-    }
     const uint64_t code = fa.code(ak[gstart[g]]);
-    const long long s = lower_code(bk, nb, code, fb);
+    const long long b0 = bound[2 * blockIdx.x];
+    const long long s = b0 + lower_code(bk + b0, bound[2 * blockIdx.x + 1] - b0, code, fb);
     // first entry past the code: gallop to a bracket [lo, hi), then bisect
     long long lo = s, step = 1, hi = s;
     while (hi < nb && fb.code(bk[hi]) == code) {

@@ -252,39 +252,57 @@ __device__ int handle_redundancies(MPath* am, int n, MPath* bm, uint16_t* T, lon
 // AUX: the instantiation that also fills job_work / redund (capture mode and
 // the branch statistics); the default one keeps those pointers out of its
 // register budget.
+// The kernel's arguments read where they are used: a scalar load through the
+// kernarg segment pointer, made opaque to the compiler so that no argument is
+// held in (or spilled from) SGPRs across the aligner's loop.
+#ifndef MI_CPU_WAVE_EMU
+typedef const ReportArgs __attribute__((address_space(4))) KernArgs;
+__device__ __forceinline__ KernArgs* kargs()
+{
+    KernArgs* p = (KernArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return p;
+}
+#else
+// the CPU wave emulator (tests/wave_emu) hands the launch's arguments over
+typedef const ReportArgs KernArgs;
+inline KernArgs* kargs() { return (KernArgs*)emu::kernarg(); }
+#endif
+#define KA(f) (kargs()->f)
+
 template <int NCH, bool AUX>
-__global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
+__global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs)
 {
     const int lane = __lane_id();
     const int wave = blockIdx.x;
-    const int W3 = 3 * a.w + 2;
-    int* F = a.bands + (size_t)wave * W3;
-    int* score = F + 1 - a.mindiag;             // score[band], band in [mindiag-1, maxdiag+1]
-    int* lastp = score + a.w;
-    int* lasta = lastp + a.w;
-    int* const edge = F + 2 * a.w;              // lastp[maxdiag] == lasta[mindiag-1]
-    Peb* pool = a.pools + (size_t)wave * a.pool_cap;
-    Peb* pts = a.pts + (size_t)wave * 2 * a.pts_cap;
-    uint16_t* latr = a.latr + (size_t)wave * 4 * a.maxtp_all;
-    MPath* am = a.mpaths + (size_t)wave * 2 * a.mcap;
-    MPath* bm = am + a.mcap;
-    uint16_t* T = a.mtrace + (size_t)wave * a.mtcap;
-    const int TS = a.tspace;
+    const int W3 = 3 * KA(w) + 2;
+    int* F = KA(bands) + (size_t)wave * W3;
+    int* score = F + 1 - KA(mindiag);             // score[band], band in [mindiag-1, maxdiag+1]
+    int* lastp = score + KA(w);
+    int* lasta = lastp + KA(w);
+    int* const edge = F + 2 * KA(w);              // lastp[maxdiag] == lasta[mindiag-1]
+    Peb* pool = KA(pools) + (size_t)wave * KA(pool_cap);
+    Peb* pts = KA(pts) + (size_t)wave * 2 * KA(pts_cap);
+    uint16_t* latr = KA(latr) + (size_t)wave * 4 * KA(maxtp_all);
+    MPath* am = KA(mpaths) + (size_t)wave * 2 * KA(mcap);
+    MPath* bm = am + KA(mcap);
+    uint16_t* T = KA(mtrace) + (size_t)wave * KA(mtcap);
+    const int TS = KA(tspace);
 
     for (;;) {
         // all lanes take part in the atomic (lane 0 adds 1), so the loop exit is uniform
-        const int t = __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, lane == 0 ? 1 : 0));
-        const int ab = __builtin_amdgcn_readfirstlane(__atomic_load_n(a.abort_flag, __ATOMIC_RELAXED));
-        if (t >= a.norder || ab) break;
-        const int job = a.order[t];
-        const int p = a.jobs[job];
-        const long long ps = a.pstart[p], pe = a.pstart[p + 1];
-        const int ar = a.hits[ps].aread, br = a.hits[ps].bread;
-        const int alen = a.lenA[ar], blen = a.lenB[br];
-        const uint64_t* Aseq = a.pkA + a.w0A[ar];
-        const uint64_t* Bseq = a.pkB + a.w0B[br];
-        const bool doA = alen >= a.hgap_min;
-        const bool doB = a.symmetric && blen >= a.hgap_min && (ar != br || !a.self || !a.comp);
+        const int t = __builtin_amdgcn_readfirstlane(atomicAdd(KA(counter), lane == 0 ? 1 : 0));
+        const int ab = __builtin_amdgcn_readfirstlane(__atomic_load_n(KA(abort_flag), __ATOMIC_RELAXED));
+        if (t >= KA(norder) || ab) break;
+        const int job = KA(order)[t];
+        const int p = KA(jobs)[job];
+        const long long ps = KA(pstart)[p], pe = KA(pstart)[p + 1];
+        const int ar = KA(hits)[ps].aread, br = KA(hits)[ps].bread;
+        const int alen = KA(lenA)[ar], blen = KA(lenB)[br];
+        const uint64_t* Aseq = KA(pkA) + KA(w0A)[ar];
+        const uint64_t* Bseq = KA(pkB) + KA(w0B)[br];
+        const bool doA = alen >= KA(hgap_min);
+        const bool doB = KA(symmetric) && blen >= KA(hgap_min) && (ar != br || !KA(self) || !KA(comp));
         const int maxtp = 2 * ((alen > blen ? alen : blen) / TS + 2);
         uint16_t* atr = latr + maxtp;
         uint16_t* btr = latr + 3 * maxtp;
@@ -296,7 +314,7 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
         long long aligned = 0, edges = 0;
         WaveCount wc;
         {
-            const int e0 = a.edge_in ? a.edge_in[job] : 0;
+            const int e0 = KA(edge_in) ? KA(edge_in)[job] : 0;
             if (lane == 0) *edge = e0;
             wave_sync();
         }
@@ -313,7 +331,7 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
             long long pend = pe, h2 = lidx;
             for (long long c = lidx; c < pe; c += 64) {
                 const long long f = c + lane;
-                const int ap = (f < pe) ? a.hits[f].apos : INT32_MAX;
+                const int ap = (f < pe) ? KA(hits)[f].apos : INT32_MAX;
                 const unsigned long long beyond = __ballot(f < pe && ap > amark);
                 const unsigned long long within2 = __ballot(f < pe && ap <= amark2);
                 if (within2) h2 = c + highest_lane(within2) + 1;
@@ -323,18 +341,18 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                 }
             }
             nidx = pend;
-            if (nidx - lidx >= a.minhit) {
+            if (nidx - lidx >= KA(minhit)) {
                 // band scores (dalign/filter.c:1757-1766): each seed adds
                 // min(K, apos - apos of the band's previous seed)
                 for (long long c = lidx; c < nidx; c += 64) {
                     const long long f = c + lane;
                     const bool v = f < nidx;
-                    int ap = 0, band = a.mindiag;
+                    int ap = 0, band = KA(mindiag);
                     if (v) {
-                        ap = a.hits[f].apos;
-                        band = a.hits[f].diag >> a.binshift;
+                        ap = KA(hits)[f].apos;
+                        band = KA(hits)[f].diag >> KA(binshift);
                     }
-                    const unsigned bi = (unsigned)(band - a.mindiag);
+                    const unsigned bi = (unsigned)(band - KA(mindiag));
                     unsigned long long peers = __ballot(v);
 #pragma unroll
                     for (int b = 0; b < 16; b++) {
@@ -346,12 +364,12 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                     const int pap = __shfl(ap, prevl, 64);
                     int prev = 0;
                     if (v) prev = below ? pap : lastp[band];
-                    if (__any(v && band == a.maxdiag)) touched = true;
-                    edges += __popcll(__ballot(v && (band == a.mindiag || band == a.maxdiag)));
+                    if (__any(v && band == KA(maxdiag))) touched = true;
+                    edges += __popcll(__ballot(v && (band == KA(mindiag) || band == KA(maxdiag))));
                     wave_sync();
                     if (v) {
                         const int d = ap - prev;
-                        atomicAdd(&score[band], d >= a.K ? a.K : d);
+                        atomicAdd(&score[band], d >= KA(K) ? KA(K) : d);
                         if (lane == highest_lane(peers)) lastp[band] = ap;
                     }
                     wave_sync();
@@ -366,9 +384,9 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                     const bool v = f < nidx;
                     bool go = false;
                     if (v) {
-                        const int ap = a.hits[f].apos, band = a.hits[f].diag >> a.binshift;
-                        go = ap > lasta[band] && (score[band] + score[band + 1] >= a.hitmin ||
-                                                  score[band] + score[band - 1] >= a.hitmin);
+                        const int ap = KA(hits)[f].apos, band = KA(hits)[f].diag >> KA(binshift);
+                        go = ap > lasta[band] && (score[band] + score[band + 1] >= KA(hitmin) ||
+                                                  score[band] + score[band - 1] >= KA(hitmin));
                     }
                     const unsigned long long gm = __ballot(go);
                     if (!gm) {
@@ -377,16 +395,21 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                     }
                     const long long fh = f0 + lowest_lane(gm);
                     f0 = fh + 1;
-                    const Seed sd = a.hits[fh];
-                    const int ap = sd.apos, dg = sd.diag, bp = ap - dg, band0 = dg >> a.binshift;
+                    const Seed sd = KA(hits)[fh];
+                    const int ap = sd.apos, dg = sd.diag, bp = ap - dg, band0 = dg >> KA(binshift);
                     // a read against itself (-I, self forward pass: the same bases): the
                     // wave may not reach the main diagonal (Local_Alignment's selfie
                     // bounds, dalign/align.c:1966-1985)
-                    const bool selfie = a.self && !a.comp && ar == br;
-                    LaResult r = local_alignment<NCH>(a.sp, Aseq, alen, Bseq, blen, a.comp, dg, dg, ap + bp,
+                    const bool selfie = KA(self) && !KA(comp) && ar == br;
+                    DevSpec spv;   // by value: the aligner reads the spec from SGPRs
+                    spv.table = KA(sp.table);
+                    spv.score = KA(sp.score);
+                    spv.tspace = KA(sp.tspace);
+                    spv.ave_path = KA(sp.ave_path);
+                    LaResult r = local_alignment<NCH>(spv, Aseq, alen, Bseq, blen, KA(comp), dg, dg, ap + bp,
                                                       (selfie && dg >= 0) ? 1 : -INT32_MAX,
-                                                      (selfie && dg <= 0) ? -1 : INT32_MAX, pool, a.pool_cap, pts,
-                                                      a.pts_cap, atr, btr, maxtp, a.abort_flag, wc);
+                                                      (selfie && dg <= 0) ? -1 : INT32_MAX, pool, KA(pool_cap), pts,
+                                                      KA(pts_cap), atr, btr, maxtp, KA(abort_flag), wc);
                     if (r.status != LA_OK) {
                         pstatus = (r.status == LA_WIDE || r.status == LA_POOL || r.status == LA_ABORT) ? RP_RETRY
                                                                                                         : RP_UNDEF;
@@ -408,26 +431,26 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                             if (dd < lo) lo = dd;
                             else if (dd > hi) hi = dd;
                         }
-                        lo = (lo >> a.binshift) - 1;
-                        hi = (hi >> a.binshift) + 1;
+                        lo = (lo >> KA(binshift)) - 1;
+                        hi = (hi >> KA(binshift)) + 1;
                         if (band0 < lo) lo = band0;
                         else if (band0 > hi) hi = band0;
                     }
-                    if (lo < a.mindiag || hi > a.maxdiag) edges++;
-                    if (lo < a.mindiag) touched = true;
+                    if (lo < KA(mindiag) || hi > KA(maxdiag)) edges++;
+                    if (lo < KA(mindiag)) touched = true;
                     // a consistent trace keeps the span within one band of the
                     // array on each side; the clamp only guards the memory
-                    lo = max(lo, a.mindiag - 1);
-                    hi = min(hi, a.maxdiag + 1);
+                    lo = max(lo, KA(mindiag) - 1);
+                    hi = min(hi, KA(maxdiag) + 1);
                     tmin = min(tmin, lo);
                     tmax = max(tmax, hi);
                     wave_sync();
                     for (int i = lane; i <= hi - lo; i += 64)
                         if (r.aepos > lasta[lo + i]) lasta[lo + i] = r.aepos;
                     wave_sync();
-                    if ((r.aepos - r.abpos) + (r.bepos - r.bbpos) >= a.minover) {
+                    if ((r.aepos - r.abpos) + (r.bepos - r.bbpos) >= KA(minover)) {
                         const long long need = (doA ? r.atlen : 0) + (doB ? r.btlen : 0);
-                        if (novla >= a.mcap || novlb >= a.mcap || ttop + need > a.mtcap) {
+                        if (novla >= KA(mcap) || novlb >= KA(mcap) || ttop + need > KA(mtcap)) {
                             pstatus = RP_ARENA;
                             break;
                         }
@@ -453,7 +476,7 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                 for (long long c = lidx; c < nidx; c += 64) {
                     const long long f = c + lane;
                     if (f < nidx) {
-                        const int band = a.hits[f].diag >> a.binshift;
+                        const int band = KA(hits)[f].diag >> KA(binshift);
                         score[band] = 0;
                         lastp[band] = 0;
                     }
@@ -467,7 +490,7 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
         // a seed band of this pair, so the reference's walk clears exactly the
         // raised in-range cells; cells outside the range keep their value
         if (tmin <= tmax) {   // some alignment raised cells (tmin / tmax start empty)
-            const int lo = max(tmin, a.mindiag), hi = min(tmax, a.maxdiag);
+            const int lo = max(tmin, KA(mindiag)), hi = min(tmax, KA(maxdiag));
             for (int i = lane; i <= hi - lo; i += 64) lasta[lo + i] = 0;
             wave_sync();
         }
@@ -478,13 +501,13 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
             if (lane == 0) {
                 unsigned br[kRedundBranches] = {};
                 if (na > 1) {
-                    if (nb > 1) na = nb = handle_redundancies(am, na, bm, T, &ttop, a.mtcap, TS, a.comp, &ok, br);
-                    else na = handle_redundancies(am, na, nullptr, T, &ttop, a.mtcap, TS, a.comp, &ok, br);
+                    if (nb > 1) na = nb = handle_redundancies(am, na, bm, T, &ttop, KA(mtcap), TS, KA(comp), &ok, br);
+                    else na = handle_redundancies(am, na, nullptr, T, &ttop, KA(mtcap), TS, KA(comp), &ok, br);
                 } else if (nb > 1)
-                    nb = handle_redundancies(bm, nb, nullptr, T, &ttop, a.mtcap, TS, a.comp, &ok, br);
+                    nb = handle_redundancies(bm, nb, nullptr, T, &ttop, KA(mtcap), TS, KA(comp), &ok, br);
                 if constexpr (AUX) {
-                    if (a.redund)   // the job's last successful run overwrites earlier ones
-                        for (int q = 0; q < kRedundBranches; q++) a.redund[(size_t)job * kRedundBranches + q] = br[q];
+                    if (KA(redund))   // the job's last successful run overwrites earlier ones
+                        for (int q = 0; q < kRedundBranches; q++) KA(redund)[(size_t)job * kRedundBranches + q] = br[q];
                 }
             }
             ok = __builtin_amdgcn_readfirstlane(ok ? 1 : 0) != 0;
@@ -496,13 +519,13 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                 long long tl = 0;
                 for (int i = 0; i < na; i++) tl += am[i].tlen;
                 for (int i = 0; i < nb; i++) tl += bm[i].tlen;
-                const int tbytes = a.small_trace ? 1 : 2;
+                const int tbytes = KA(small_trace) ? 1 : 2;
                 unsigned long long r0 = 0, t0 = 0;
-                r0 = atomicAdd(&a.ocount[0], lane == 0 ? (unsigned long long)(na + nb) : 0ull);
-                t0 = atomicAdd(&a.ocount[1], lane == 0 ? (unsigned long long)(tl * tbytes) : 0ull);
+                r0 = atomicAdd(&KA(ocount)[0], lane == 0 ? (unsigned long long)(na + nb) : 0ull);
+                t0 = atomicAdd(&KA(ocount)[1], lane == 0 ? (unsigned long long)(tl * tbytes) : 0ull);
                 const unsigned long long rbase = __shfl(r0, 0, 64);
                 const unsigned long long tbase = __shfl(t0, 0, 64);
-                if (rbase + na + nb > (unsigned long long)a.ocap || tbase + tl * tbytes > (unsigned long long)a.otcap)
+                if (rbase + na + nb > (unsigned long long)KA(ocap) || tbase + tl * tbytes > (unsigned long long)KA(otcap))
                     pstatus = RP_OUT;
                 else {
                     unsigned long long to = tbase;
@@ -517,23 +540,23 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
                             o.bbpos = m.bbpos;
                             o.aepos = m.aepos;
                             o.bepos = m.bepos;
-                            o.flags = (uint32_t)a.comp;
-                            o.aread = isA ? ar + a.afirst : br + a.bfirst;
-                            o.bread = isA ? br + a.bfirst : ar + a.afirst;
+                            o.flags = (uint32_t)KA(comp);
+                            o.aread = isA ? ar + KA(afirst) : br + KA(bfirst);
+                            o.bread = isA ? br + KA(bfirst) : ar + KA(afirst);
                             o.local = isA ? i : i - na;
                             o.job = job;
                             o.toff = (long long)to;
-                            o.gen = a.gen;
+                            o.gen = KA(gen);
                             o.pad = 0;
-                            a.orecs[rbase + i] = o;
+                            KA(orecs)[rbase + i] = o;
                         }
                         // Compress_TraceTo8 (dalign/align.c:3352-3360) when the spacing is <= 125
                         for (int q = lane; q < m.tlen; q += 64) {
                             const uint16_t x = T[m.toff + q];
-                            if (tbytes == 1) a.otr[to + q] = (uint8_t)x;
+                            if (tbytes == 1) KA(otr)[to + q] = (uint8_t)x;
                             else {
-                                a.otr[to + 2 * q] = (uint8_t)(x & 0xff);
-                                a.otr[to + 2 * q + 1] = (uint8_t)(x >> 8);
+                                KA(otr)[to + 2 * q] = (uint8_t)(x & 0xff);
+                                KA(otr)[to + 2 * q + 1] = (uint8_t)(x >> 8);
                             }
                         }
                         to += (unsigned long long)m.tlen * tbytes;
@@ -544,18 +567,18 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs a)
         wave_sync();
         const int eout = *edge;
         if (lane == 0) {
-            a.status[job] = pstatus;
-            if (pstatus == RP_OK) a.final_gen[job] = a.gen;
-            a.edge_out[job] = eout;
-            a.touch[job] = touched ? 1 : 0;
+            KA(status)[job] = pstatus;
+            if (pstatus == RP_OK) KA(final_gen)[job] = KA(gen);
+            KA(edge_out)[job] = eout;
+            KA(touch)[job] = touched ? 1 : 0;
             if (pstatus == RP_OK) {
-                atomicAdd(&a.stats[0], (unsigned long long)aligned);
-                atomicAdd(&a.stats[1], (unsigned long long)wc.cells);
-                atomicAdd(&a.stats[2], (unsigned long long)wc.snake);
-                atomicAdd(&a.stats[3], (unsigned long long)edges);
-                atomicAdd(&a.stats[4], (unsigned long long)wc.steps);
+                atomicAdd(&KA(stats)[0], (unsigned long long)aligned);
+                atomicAdd(&KA(stats)[1], (unsigned long long)wc.cells);
+                atomicAdd(&KA(stats)[2], (unsigned long long)wc.snake);
+                atomicAdd(&KA(stats)[3], (unsigned long long)edges);
+                atomicAdd(&KA(stats)[4], (unsigned long long)wc.steps);
                 if constexpr (AUX) {
-                    if (a.job_work) a.job_work[job] = (unsigned long long)(wc.cells + wc.snake);
+                    if (KA(job_work)) KA(job_work)[job] = (unsigned long long)(wc.cells + wc.snake);
                 }
             }
         }

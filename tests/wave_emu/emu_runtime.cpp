@@ -14,6 +14,7 @@ extern "C" float __log2f(float x) noexcept { return log2f(x); }
 namespace emu {
 thread_local Ctx* g = nullptr;
 thread_local int lane = 0;
+thread_local const void* t_kernarg = nullptr;
 thread_local dim3 t_block, t_thread;
 dim3 g_bdim, g_gdim;
 YieldBarrier* g_blockbar = nullptr;

@@ -72,7 +72,19 @@ void launch(dim3 grid, dim3 block, F&& body)
 #define blockDim emu::g_bdim
 #define gridDim emu::g_gdim
 #define __syncthreads() emu::g_blockbar->arrive_and_wait()
-#define hipLaunchKernelGGL(K, G, B, S, ST, ...) emu::launch((G), (B), [&]() { K(__VA_ARGS__); })
+namespace emu {
+extern thread_local const void* t_kernarg;
+// a kernel called with copies of its arguments; the first one's address
+// stands for the kernarg segment (kernels that read their arguments there)
+template <class F, class A0, class... As>
+inline void call_kernel(F&& f, A0 a0, As... as)
+{
+    t_kernarg = &a0;
+    f(a0, as...);
+}
+inline const void* kernarg() { return t_kernarg; }
+}  // namespace emu
+#define hipLaunchKernelGGL(K, G, B, S, ST, ...) emu::launch((G), (B), [&]() { emu::call_kernel(K, __VA_ARGS__); })
 
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }

@@ -200,7 +200,7 @@ int mi_set_tuning(const MiTuning* t)
                     (t->report_l1_waves_per_cu == 0 || (t->report_l1_waves_per_cu >= 2 && t->report_l1_waves_per_cu <= 32)) &&
                     (t->batch_waves_per_cu == 0 || (t->batch_waves_per_cu >= 4 && t->batch_waves_per_cu <= 32)) &&
                     (t->batch_wide_waves_per_cu == 0 ||
-                     (t->batch_wide_waves_per_cu >= 2 && t->batch_wide_waves_per_cu <= 20));
+                     (t->batch_wide_waves_per_cu >= 2 && t->batch_wide_waves_per_cu <= 32));
     if (!ok) {
         mi::mi_set_error("mi_set_tuning: value out of range");
         return MI_ERR_ARG;

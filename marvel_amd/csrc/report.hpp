@@ -594,5 +594,6 @@ __global__ __launch_bounds__(64, NCH == 1 ? 8 : 1) void k_report(ReportArgs)
         wave_sync();
     }
 }
+#undef KA
 
 }  // namespace mi

@@ -1,9 +1,12 @@
 // DBdust on the GPU: the SDUST low-complexity scan of db/DBdust.c:236-470
-// (Morgulis et al., JCB 13(5) 2006) over every read of a block, one lane per
-// read.  The scan is sequential along a read, so the parallelism is across
-// reads; each lane keeps its triplet counts and (for windows up to 64) its
-// candidate list in LDS ([item][lane], a bank per lane), its interval stack
-// in global memory.
+// (Morgulis et al., JCB 13(5) 2006) over every read of a block.  The scan is
+// sequential along a read.  With the reference's unbiased scores (the
+// default) each read is cut into segments of 384 steps, one lane each, and
+// a lane reaches its segment's true starting state by a warm-up (the
+// segmented scan below); the biased scores (-b) and windows below 3 keep one
+// lane per read.  Each lane keeps its triplet counts and a ring of triplet
+// codes in LDS ([item][lane], a bank per lane); the candidate lists sit in
+// LDS (sequential scan, windows up to 64) or HBM.
 //
 // Per read (triplet codes t[0 .. len-3] of bases j, j+1, j+2):
 //   * the window counts cover (wb, j], the scanned run (lb, j]; wsqr / lsqr
@@ -281,9 +284,524 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
     }
 }
 
+// ---- segmented scan (unbiased scores) ------------------------------------------
+// Before step j the unbiased scan's state is (lb, the candidate list): wb is
+// max(-1, j - 1 - window + 2), both count arrays and their sums follow from
+// wb, lb and the read, and every live candidate begins inside the window.
+// So a read is cut into segments of `seglen` steps, one lane each.  A
+// segment's lane starts `warm` steps early from the smallest state (lb = wb,
+// no candidate; lb only ever moves forward and the map from the previous lb
+// to the next is monotone, so the two runs meet once the true run's cuts
+// are inside the warm-up), records the state it reached at its first step,
+// then runs its own steps, folding the intervals it retires as the interval
+// stack does and appending them, and records its final state.  A second
+// kernel compares each segment's first state with its predecessor's final
+// state.  Where they differ, one lane per read re-runs from the true state
+// and carries on until a final state matches the next segment's first.
+// Retirement order is increasing begin (one retirement per step, of the
+// tail, whose begin is then wb), so folding each segment's intervals and
+// then the segments' results in order gives the sequential stack.  The
+// biased scores are sums of doubles whose rounding depends on the whole
+// read, so they keep the sequential kernel above.
+
+struct DustEnt {
+    int beg, end;
+    double sc;
+};
+struct DustHdr {
+    int lb, n;
+};
+struct DustIv {   // a folded interval: segment, pass (0 main, 1 repair), begin, end + 2
+    int seg, pass, b, e;
+};
+
+struct DustSegArgs {
+    const uint8_t* bases;
+    const long long* boff;
+    const int* rlen;
+    const int* seg_read;   // [nseg] read of the segment (segments of a read are consecutive)
+    const int* seg_s0;     // [nseg] its first step
+    int nseg, seglen, warm, window, thresh2i;
+    double thresh;
+    // candidate nodes, [node][slot], slot < nstride
+    int* cbeg;
+    int* cend;
+    double* csc;
+    short* cnx;
+    short* cpv;
+    long long nstride;
+    // boundary states, record 2s = the state segment s began its own steps
+    // in, 2s + 1 = the state after its last step: hdr = (lb, n) and n
+    // candidates (head first, decreasing begin) at ent[rec * (window + 1)]
+    DustHdr* hdr;
+    DustEnt* ent;
+    uint8_t* flag;       // compare: segment s began in another state than s - 1 ended in
+    uint8_t* repaired;   // repair: segments re-run
+    const int* rep_seg;  // repair: per listed read, its first flagged segment
+    int nrep;
+    int* mismatch;       // compare: flagged segments
+    DustIv* out;         // the folded intervals
+    int* out_cnt;
+    int out_cap;
+    int* bad;            // per segment: a candidate list outgrew its nodes
+};
+
+__device__ __forceinline__ bool dust_rec_equal(const DustSegArgs& a, long long x, long long y)
+{
+    const DustHdr hx = a.hdr[x], hy = a.hdr[y];
+    if (hx.lb != hy.lb || hx.n != hy.n) return false;
+    const int E = a.window + 1;
+    for (int i = 0; i < hx.n; i++) {   // scores are finite and positive: == is bit equality
+        const DustEnt ex = a.ent[x * E + i], ey = a.ent[y * E + i];
+        if (ex.beg != ey.beg || ex.end != ey.end || ex.sc != ey.sc) return false;
+    }
+    return true;
+}
+
+// RING: triplet-code ring per lane, at least 2 x window slots.  REPAIR:
+// one lane per listed read, from its first flagged segment.
+template <int RING, bool REPAIR>
+__global__ __launch_bounds__(64) void k_sdust_seg(DustSegArgs a)
+{
+    __shared__ uint8_t wcnt[64 * 64], lcnt[64 * 64];   // [triplet][lane]
+    __shared__ uint8_t ring[RING * 64];                // [slot][lane]
+    const int lane = threadIdx.x;
+    const long long slot = (long long)blockIdx.x * 64 + lane;
+    const long long S = a.nstride;
+    const int W = a.window, E = W + 1;
+    auto BEG = [&](int n) -> int& { return a.cbeg[n * S + slot]; };
+    auto END = [&](int n) -> int& { return a.cend[n * S + slot]; };
+    auto SC = [&](int n) -> double& { return a.csc[n * S + slot]; };
+    auto NX = [&](int n) -> short& { return a.cnx[n * S + slot]; };
+    auto PV = [&](int n) -> short& { return a.cpv[n * S + slot]; };
+    auto WC = [&](int e) -> uint8_t& { return wcnt[e * 64 + lane]; };
+    auto LC = [&](int e) -> uint8_t& { return lcnt[e * 64 + lane]; };
+    auto T = [&](int i) { return (int)ring[(i & (RING - 1)) * 64 + lane]; };
+    if (slot >= (REPAIR ? a.nrep : a.nseg)) return;
+    int s = REPAIR ? a.rep_seg[slot] : (int)slot;
+    const int r = a.seg_read[s];
+    const uint8_t* b = a.bases + a.boff[r];
+    const int L = a.rlen[r] - 2;
+    int j = 0, wb = -1, lb = -1, wsqr = 0, lsqr = 0;
+    int freel = 1, tail = 0, tailbeg = 0;
+    int topb = 0, tope = 0;   // the segment's top interval, not yet appended
+    bool has = false;
+    // codes of positions [lo, min(hi, L)) into the ring
+    auto fill = [&](int lo, int hi) {
+        hi = min(hi, L);
+        if (lo >= hi) return;
+        int c = (b[lo] << 2) | b[lo + 1];
+        for (int i = lo; i < hi; i++) {
+            c = ((c << 2) & 63) | b[i + 2];
+            ring[(i & (RING - 1)) * 64 + lane] = (uint8_t)c;
+        }
+    };
+    // the state before step p: lb and the candidates of record `rec`, or
+    // (rec < 0) lb = wb and no candidate
+    auto setup = [&](int p, long long rec) {
+        j = p;
+        wb = (p - 1 > W - 3) ? p - W + 1 : -1;
+        lb = rec < 0 ? wb : a.hdr[rec].lb;
+        const int n = rec < 0 ? 0 : a.hdr[rec].n;
+        fill(max(0, wb + 1), (p + RING / 2 - 1) & ~(RING / 2 - 1));
+        for (int e = 0; e < 64; e++) WC(e) = LC(e) = 0;
+        wsqr = lsqr = 0;
+        for (int i = wb + 1; i < p; i++) wsqr += WC(T(i))++;
+        for (int i = lb + 1; i < p; i++) lsqr += LC(T(i))++;
+        BEG(0) = -2;
+        NX(0) = (short)(n ? 1 : 0);
+        PV(0) = (short)n;
+        for (int i = 0; i < n; i++) {
+            const DustEnt e = a.ent[rec * E + i];
+            BEG(i + 1) = e.beg;
+            END(i + 1) = e.end;
+            SC(i + 1) = e.sc;
+            NX(i + 1) = (short)(i + 1 < n ? i + 2 : 0);
+            PV(i + 1) = (short)i;
+        }
+        for (int k = n + 1; k < W; k++) NX(k) = (short)(k + 1);
+        if (n < W) NX(W) = -1;
+        freel = n < W ? n + 1 : -1;
+        tail = n;
+        tailbeg = n ? a.ent[rec * E + n - 1].beg : 0;
+    };
+    auto save = [&](long long rec) {
+        int n = 0;
+        for (int J = NX(0); J != 0 && n < E; J = NX(J), n++) a.ent[rec * E + n] = DustEnt{BEG(J), END(J), SC(J)};
+        a.hdr[rec] = DustHdr{lb, n};
+    };
+    auto emit = [&]() {
+        const int k = atomicAdd(a.out_cnt, 1);
+        if (k < a.out_cap) a.out[k] = DustIv{s, REPAIR ? 1 : 0, topb, tope};
+    };
+    // retire the tail candidate (its begin has left the window), folding it
+    // into the top interval when `rec`
+    auto retire = [&](bool rec) {
+        const int J = tail;
+        if (rec) {
+            const int bb = tailbeg, e = END(J) + 2;
+            if (has && tope + 1 >= bb) {
+                if (tope < e) tope = e;
+            } else {
+                if (has) emit();
+                topb = bb;
+                tope = e;
+                has = true;
+            }
+        }
+        const int p = PV(J);
+        PV(0) = (short)p;
+        NX(p) = 0;
+        NX(J) = (short)freel;
+        freel = J;
+        tail = p;
+        tailbeg = p ? BEG(p) : 0;
+    };
+    // one step of db/DBdust.c:386-466; false when a list outgrew its nodes
+    auto step = [&](bool rec) -> bool {
+        if ((j & (RING / 2 - 1)) == 0) fill(j, j + RING / 2);
+        int c = T(j);
+        if (j > W - 3) {
+            const int d = T(++wb);
+            wsqr -= --WC(d);
+        }
+        wsqr += WC(c)++;
+        if (lb < wb) {
+            const int d = T(++lb);
+            lsqr -= --LC(d);
+        }
+        const int trun = LC(c)++;
+        lsqr += trun;
+        // the loops over the run read the next position's code one iteration
+        // ahead (ring and counts are separate LDS arrays, so the read overlaps
+        // the count's round trip; the ring index wraps, so it stays in bounds)
+        if (trun >= a.thresh2i)
+            for (int dn = T(lb + 1); lb < j;) {
+                const int d = dn;
+                dn = T(++lb + 1);
+                lsqr -= --LC(d);
+                if (d == c) break;
+            }
+        if (tail != 0 && tailbeg <= wb) retire(rec);
+        if (wsqr <= lsqr * a.thresh) {
+            j++;
+            return true;
+        }
+        int J = NX(0), Lp = 0;
+        int jbeg = J ? BEG(J) : -2, lpbeg = -2;
+        double mscore = 0.;
+        int dn = T(lb);
+        for (c = lb; c > wb; c--) {
+            const int d = dn;
+            dn = T(c - 1);
+            lsqr += LC(d)++;
+            if (lsqr >= a.thresh * (j - c)) {
+                for (int g = 0; jbeg >= c;) {
+                    if (++g > W + 1) return false;
+                    const double sc = SC(J);
+                    if (sc > mscore) mscore = sc;
+                    Lp = J;
+                    lpbeg = jbeg;
+                    J = NX(J);
+                    jbeg = J ? BEG(J) : -2;
+                }
+                if (lsqr >= mscore * (j - c)) {
+                    mscore = (1. * lsqr) / (j - c);
+                    if (lpbeg == c) {
+                        END(Lp) = j;
+                        SC(Lp) = mscore;
+                    } else {
+                        const int A = freel;
+                        if (A < 0) return false;
+                        freel = NX(A);
+                        BEG(A) = c;
+                        END(A) = j;
+                        SC(A) = mscore;
+                        PV(A) = (short)Lp;
+                        NX(Lp) = (short)A;
+                        PV(J) = (short)A;
+                        NX(A) = (short)J;
+                        if (J == 0) {
+                            tail = A;
+                            tailbeg = c;
+                        }
+                        Lp = A;
+                        lpbeg = c;
+                    }
+                }
+            }
+        }
+        for (dn = T(++c); c <= lb; c++) {
+            const int d = dn;
+            dn = T(c + 1);
+            lsqr -= --LC(d);
+        }
+        j++;
+        return true;
+    };
+    auto drain = [&]() -> bool {
+        for (int g = 0; tail != 0 && g <= W; g++) retire(true);
+        return tail == 0;
+    };
+    if (!REPAIR) {
+        const int s0 = a.seg_s0[s], s1 = min(s0 + a.seglen, L);
+        setup(s0 == 0 ? 0 : max(0, s0 - a.warm), -1);
+        bool ok = true;
+        while (ok && j < s0) ok = step(false);
+        if (s0 > 0) save(2LL * s);
+        while (ok && j < s1) ok = step(true);
+        if (ok) {
+            if (s1 == L) ok = drain();
+            else save(2LL * s + 1);
+        }
+        if (has) emit();
+        if (!ok) a.bad[s] = 1;
+        return;
+    }
+    // re-run from the true final state of the segment before
+    setup(a.seg_s0[s], 2LL * (s - 1) + 1);
+    for (;;) {
+        const int s1 = min(a.seg_s0[s] + a.seglen, L);
+        a.repaired[s] = 1;
+        has = false;
+        bool ok = true;
+        while (ok && j < s1) ok = step(true);
+        if (ok && s1 == L) ok = drain();
+        if (has) emit();
+        if (!ok) {
+            a.bad[s] = 1;
+            return;
+        }
+        if (s1 == L) return;
+        save(2LL * s + 1);
+        if (!dust_rec_equal(a, 2LL * s + 1, 2LL * (s + 1))) {
+            s++;   // carry on into the next segment with the state in hand
+            continue;
+        }
+        int t = s + 2;   // the next segment that began in another state than its predecessor ended in
+        while (t < a.nseg && a.seg_read[t] == r && !a.flag[t]) t++;
+        if (t >= a.nseg || a.seg_read[t] != r) return;
+        s = t;
+        setup(a.seg_s0[t], 2LL * (t - 1) + 1);
+    }
+}
+
+__global__ void k_dust_compare(DustSegArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nseg) return;
+    const bool f = a.seg_s0[s] > 0 && !dust_rec_equal(a, 2LL * s, 2LL * (s - 1) + 1);
+    a.flag[s] = f ? 1 : 0;
+    if (f) atomicAdd(a.mismatch, 1);
+}
+
 }  // namespace mi
 
 using namespace mi;
+
+namespace {
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+// The segmented scan of the unbiased scores over the reads of `db`:
+// anno/points as mi_dust.  Bases already on the device at dbases (read i at
+// boff[i]).
+int dust_segmented(const HITS_DB* db, const void* dbases, const std::vector<long long>& boff,
+                   const std::vector<int>& rlen, int window, double threshold, int mlen, long long* anno, int* points)
+{
+    const int n = db->nreads;
+    const int seglen = std::max(64, env_int("MI_DUST_SEGMENT", 384));
+    const int warm = std::max(0, env_int("MI_DUST_WARMUP", std::max(64, 2 * window)));
+    std::vector<int> seg_read, seg_s0;
+    for (int i = 0; i < n; i++)
+        for (int s0 = 0; s0 < rlen[i] - 2; s0 += seglen) {
+            seg_read.push_back(i);
+            seg_s0.push_back(s0);
+        }
+    const int nseg = (int)seg_read.size();
+    if (nseg == 0) {
+        for (int i = 0; i <= n; i++) anno[i] = 0;
+        return 0;
+    }
+    const long long E = window + 1;
+    const long long nstride = (nseg + 63LL) / 64 * 64;
+    long long cap = 4LL * nseg + 4096;
+    std::vector<void*> bufs;
+    int rc = 0;
+    const char* err = "mi_dust: device error";
+    auto M = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (rc == 0 && hipMalloc(&p, std::max<size_t>(bytes, 8)) == hipSuccess) {
+            bufs.push_back(p);
+            return p;
+        }
+        if (rc == 0) err = "mi_dust: device allocation failed";
+        rc = MI_ERR_DEVICE;
+        return nullptr;
+    };
+    DustSegArgs a{};
+    a.bases = (const uint8_t*)dbases;
+    a.boff = (const long long*)M(8 * (size_t)n);
+    a.rlen = (const int*)M(4 * (size_t)n);
+    a.seg_read = (const int*)M(4 * (size_t)nseg);
+    a.seg_s0 = (const int*)M(4 * (size_t)nseg);
+    a.nseg = nseg;
+    a.seglen = seglen;
+    a.warm = warm;
+    a.window = window;
+    a.thresh = threshold;
+    a.thresh2i = (int)ceil(2. * threshold);
+    a.cbeg = (int*)M(4 * E * nstride);
+    a.cend = (int*)M(4 * E * nstride);
+    a.csc = (double*)M(8 * E * nstride);
+    a.cnx = (short*)M(2 * E * nstride);
+    a.cpv = (short*)M(2 * E * nstride);
+    a.nstride = nstride;
+    a.hdr = (DustHdr*)M(sizeof(DustHdr) * 2 * (size_t)nseg);
+    a.ent = (DustEnt*)M(sizeof(DustEnt) * 2 * E * nseg);
+    a.flag = (uint8_t*)M(nseg);
+    a.repaired = (uint8_t*)M(nseg);
+    int* counters = (int*)M(3 * sizeof(int));   // mismatches, intervals, unused
+    a.bad = (int*)M(4 * (size_t)nseg);
+    if (rc == 0) {
+        a.mismatch = counters;
+        a.out_cnt = counters + 1;
+        const bool ok =
+            hipMemcpy((void*)a.boff, boff.data(), 8 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy((void*)a.rlen, rlen.data(), 4 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy((void*)a.seg_read, seg_read.data(), 4 * (size_t)nseg, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy((void*)a.seg_s0, seg_s0.data(), 4 * (size_t)nseg, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) rc = MI_ERR_DEVICE;
+    }
+    const bool large = window > 64;
+    std::vector<DustIv> out;
+    std::vector<uint8_t> repaired;
+    int mism = 0, nrep = 0;
+    for (int attempt = 0; rc == 0 && attempt < 2; attempt++) {
+        a.out = (DustIv*)M(sizeof(DustIv) * (size_t)cap);
+        a.out_cap = (int)std::min<long long>(cap, INT32_MAX);
+        if (rc) break;
+        if (hipMemset(counters, 0, 3 * sizeof(int)) != hipSuccess ||
+            hipMemset(a.bad, 0, 4 * (size_t)nseg) != hipSuccess ||
+            hipMemset(a.repaired, 0, nseg) != hipSuccess) {
+            rc = MI_ERR_DEVICE;
+            break;
+        }
+        const dim3 grid((unsigned)(nstride / 64));
+        if (large) hipLaunchKernelGGL((k_sdust_seg<512, false>), grid, dim3(64), 0, 0, a);
+        else hipLaunchKernelGGL((k_sdust_seg<128, false>), grid, dim3(64), 0, 0, a);
+        hipLaunchKernelGGL(k_dust_compare, dim3((unsigned)((nseg + 255) / 256)), dim3(256), 0, 0, a);
+        int cnt[2];
+        if (hipGetLastError() != hipSuccess || hipMemcpy(cnt, counters, 2 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = MI_ERR_DEVICE;
+            break;
+        }
+        mism = cnt[0];
+        nrep = 0;
+        void* drep = nullptr;
+        if (mism > 0) {
+            // one repair lane per read with a flagged segment, from its first
+            std::vector<uint8_t> flag(nseg);
+            if (hipMemcpy(flag.data(), a.flag, nseg, hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = MI_ERR_DEVICE;
+                break;
+            }
+            std::vector<int> rep;
+            for (int s = 0; s < nseg; s++)
+                if (flag[s] && (rep.empty() || seg_read[rep.back()] != seg_read[s])) rep.push_back(s);
+            nrep = (int)rep.size();
+            drep = M(4 * rep.size());
+            if (rc || hipMemcpy(drep, rep.data(), 4 * rep.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                rc = MI_ERR_DEVICE;
+                break;
+            }
+            a.rep_seg = (const int*)drep;
+            a.nrep = nrep;
+            const dim3 rgrid((unsigned)((nrep + 63) / 64));
+            if (large) hipLaunchKernelGGL((k_sdust_seg<512, true>), rgrid, dim3(64), 0, 0, a);
+            else hipLaunchKernelGGL((k_sdust_seg<128, true>), rgrid, dim3(64), 0, 0, a);
+        }
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(cnt, counters, 2 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = MI_ERR_DEVICE;
+            break;
+        }
+        if (cnt[1] > a.out_cap) {   // more intervals than the buffer: again with room for all
+            cap = (long long)cnt[1] + 4096;
+            continue;
+        }
+        std::vector<int> bad(nseg);
+        out.resize(cnt[1]);
+        repaired.resize(nseg);
+        if (hipMemcpy(bad.data(), a.bad, 4 * (size_t)nseg, hipMemcpyDeviceToHost) != hipSuccess ||
+            (cnt[1] && hipMemcpy(out.data(), a.out, sizeof(DustIv) * (size_t)cnt[1], hipMemcpyDeviceToHost) != hipSuccess) ||
+            hipMemcpy(repaired.data(), a.repaired, nseg, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = MI_ERR_DEVICE;
+            break;
+        }
+        for (int s = 0; s < nseg; s++)
+            if (bad[s]) {
+                rc = MI_ERR_ARG;
+                err = "mi_dust: a read needs more candidate intervals than the window holds";
+                break;
+            }
+        break;
+    }
+    for (void* p : bufs) (void)hipFree(p);
+    if (rc) {
+        mi_set_error(err);
+        return rc;
+    }
+    if (getenv("MI_DUST_STATS"))
+        fprintf(stderr, "mi_dust: %d segments of %d steps (warm-up %d), %d began in another state, %d reads re-run\n",
+                nseg, seglen, warm, mism, nrep);
+    // the intervals of each segment in append order (one lane appends a
+    // segment's intervals in order), from the pass that ran it last
+    std::vector<int> first(nseg + 1, 0);
+    for (const DustIv& e : out)
+        if (e.pass == repaired[e.seg]) first[e.seg + 1]++;
+    for (int s = 0; s < nseg; s++) first[s + 1] += first[s];
+    struct Iv {
+        int b, e;
+    };
+    std::vector<Iv> iv(first[nseg]);
+    std::vector<int> at(first.begin(), first.end() - 1);
+    for (const DustIv& e : out)
+        if (e.pass == repaired[e.seg]) iv[at[e.seg]++] = Iv{e.b, e.e};
+    // fold the segments' intervals read by read, as the interval stack does,
+    // and keep those of at least mlen as points (b, e + 1)
+    long long o = 0;
+    anno[0] = 0;
+    std::vector<Iv> stk;
+    for (int i = 0, s = 0; i < n; i++) {
+        stk.clear();
+        for (; s < nseg && seg_read[s] == i; s++)
+            for (int k = first[s]; k < first[s + 1]; k++) {
+                const Iv v = iv[k];
+                if (!stk.empty() && stk.back().e + 1 >= v.b) {
+                    if (stk.back().e < v.e) stk.back().e = v.e;
+                } else {
+                    stk.push_back(v);
+                }
+            }
+        for (const Iv& v : stk)
+            if (v.e - v.b >= mlen) {
+                if (points) {
+                    points[o] = v.b;
+                    points[o + 1] = v.e + 1;
+                }
+                o += 2;
+            }
+        anno[i + 1] = 4 * o;
+    }
+    return 0;
+}
+
+}  // namespace
 
 extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen, int biased, long long* anno,
                        int* points)
@@ -315,6 +833,20 @@ extern "C" int mi_dust(const HITS_DB* db, int window, double threshold, int mlen
     }
     soff[n] = tot;
     for (int i = 0; i < n; i++) boff[i] = db->reads[i].boff - lo;
+    if (!biased && window >= 3) {
+        void* dbases = nullptr;
+        if (hipMalloc(&dbases, std::max<long long>(hi - lo, 8)) != hipSuccess) {
+            mi_set_error("mi_dust: device allocation failed");
+            return MI_ERR_DEVICE;
+        }
+        int rc = MI_ERR_DEVICE;
+        if (hipMemcpy(dbases, (const char*)db->bases + lo, hi - lo, hipMemcpyHostToDevice) == hipSuccess)
+            rc = dust_segmented(db, dbases, boff, rlen, window, threshold, mlen, anno, points);
+        else
+            mi_set_error("mi_dust: device error");
+        (void)hipFree(dbases);
+        return rc;
+    }
     // the biased weights, with the reference's float product (db/DBdust.c:250-257)
     double skew[64];
     for (int p = 0, x = 0; x < 4; x++)

@@ -36,17 +36,24 @@ def stem(case: str) -> str:
     return "." + (arg[:-3] if arg.endswith(".db") else arg) + ".dust"
 
 
-def run(exe: str, case: str, d: str) -> dict:
+def run(exe: str, case: str, d: str, env: dict | None = None, log: list | None = None) -> dict:
+    """Runs `exe` on the case in directory d (extra environment `env`; the
+    runs' stderr appended to `log`) and returns the SHA-256 of the track files."""
     data, split, opts, arg, grow = CASES[case]
+    full_env = dict(os.environ, **(env or {}))
     root = os.path.join(d, {"edges": "E", "5mbp": "C2"}.get(data, "C1"))
     reads = las_cases.reads_of(data)
     if grow:
         dbio.write_db(root, reads[:grow], size_mbp=split)
-        r = subprocess.run([exe] + opts + [arg], cwd=d, capture_output=True, timeout=300)
+        r = subprocess.run([exe] + opts + [arg], cwd=d, capture_output=True, timeout=300, env=full_env)
+        if log is not None:
+            log.append(r.stderr.decode(errors="replace"))
         if r.returncode != 0:
             raise RuntimeError(f"{exe} {case} (first {grow} reads): rc {r.returncode} {r.stderr[-2000:]!r}")
     dbio.write_db(root, reads, size_mbp=split)
-    r = subprocess.run([exe] + opts + [arg], cwd=d, capture_output=True, timeout=300)
+    r = subprocess.run([exe] + opts + [arg], cwd=d, capture_output=True, timeout=300, env=full_env)
+    if log is not None:
+        log.append(r.stderr.decode(errors="replace"))
     if r.returncode != 0:
         raise RuntimeError(f"{exe} {case}: rc {r.returncode} {r.stderr[-2000:]!r}")
     return {ext: hashlib.sha256(open(os.path.join(d, stem(case) + "." + ext), "rb").read()).hexdigest()

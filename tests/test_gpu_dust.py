@@ -2,7 +2,9 @@
 .dust.anno / .dust.data as the compiled reference DBdust for every golden
 case: default and non-default window, threshold and minimum length, the
 biased scores, a DB block, a DB that grew since its last scan, and the
-5 Mbp / 30x read set."""
+5 Mbp / 30x read set.  The unbiased scan runs in segments with a warm-up
+each; short segments without warm-up force most segment boundaries through
+the re-run of the true state, and the tracks stay the same."""
 
 import json
 import os
@@ -22,6 +24,20 @@ def test_gpu_dbdust_matches_reference(case, tmp_path):
     assert os.path.exists(EXE), "DBdust not built (python -m marvel_amd.build)"
     print(f"[gpu DBdust] case {case}", flush=True)
     assert dust_cases.run(EXE, case, str(tmp_path)) == GOLD[case]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["rep_default", "rep_w100_t3", "rep_grow", "edges_default", "c2_default"])
+def test_gpu_dbdust_segment_rerun(case, tmp_path):
+    assert os.path.exists(EXE), "DBdust not built (python -m marvel_amd.build)"
+    print(f"[gpu DBdust] case {case}, 64-step segments, no warm-up", flush=True)
+    log = []
+    env = {"MI_DUST_SEGMENT": "64", "MI_DUST_WARMUP": "0", "MI_DUST_STATS": "1"}
+    assert dust_cases.run(EXE, case, str(tmp_path), env=env, log=log) == GOLD[case]
+    stats = [ln for ln in "".join(log).splitlines() if ln.startswith("mi_dust:")]
+    assert stats, log
+    rerun = int(stats[-1].split(",")[-1].split()[0])
+    assert rerun > 0, stats
 
 
 def test_dbdust_options_are_checked(tmp_path):

@@ -185,13 +185,20 @@ __device__ __forceinline__ int prefix_max_nn(int x)
 // (DPP wave_shl:1 / wave_shr:1)
 __device__ __forceinline__ int lane_above_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xF, 0xF, false); }
 __device__ __forceinline__ int lane_below_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xF, 0xF, false); }
-// inclusive max over lanes >= L (x >= -1): the lanes reversed by one
-// ds_bpermute each way around the DPP prefix scan (the LDS pipe is idle here,
-// the VALU is not)
+// inclusive max over lanes >= L (x >= -1).  GATHER: the lanes reversed by one
+// ds_bpermute each way around the DPP prefix scan, for passes that run many
+// waves per SIMD (the LDS pipe is idle there, the VALU is not); otherwise
+// the DPP suffix scan, which keeps the two LDS round trips off the path of a
+// wave that runs alone (the wider windows).
+template <bool GATHER>
 __device__ __forceinline__ int wave_suffix_max(int x)
 {
-    const int r = 63 - __lane_id();
-    return __shfl(prefix_max_nn(__shfl(x + 1, r, 64)), r, 64) - 1;
+    if constexpr (GATHER) {
+        const int r = 63 - __lane_id();
+        return __shfl(prefix_max_nn(__shfl(x + 1, r, 64)), r, 64) - 1;
+    } else {
+        return suffix_max_nn(x + 1) - 1;
+    }
 }
 // inclusive min over lanes <= L (any x): the max of INT32_MAX - x >= 0
 __device__ __forceinline__ int wave_prefix_min(int x) { return 2147483647 - prefix_max_nn(2147483647 - x); }
@@ -247,17 +254,26 @@ __device__ __forceinline__ int pick3(int v, int sel)
     const int b = lane_below(v), a = lane_above(v);
     return sel < 0 ? b : (sel > 0 ? a : v);
 }
-// The predecessor's fields the pass carries on (M, T, HA, HB): one
-// ds_bpermute per field from lane (L + sel) & 63, on the LDS pipe rather
-// than as DPP moves and selects on the VALU.  V, NA and NB stay the lane's own.
+// The predecessor's fields the pass carries on (M, T, HA, HB) from lane
+// (L + sel) & 63; V, NA and NB stay the lane's own.  GATHER: one ds_bpermute
+// per field, on the LDS pipe rather than as DPP moves and selects on the
+// VALU; otherwise the DPP moves (shorter latency for a wave that runs alone).
+template <bool GATHER>
 __device__ __forceinline__ Diag diag_pick(const Diag& D, int sel)
 {
-    const int src = (__lane_id() + sel) & 63;
     Diag g;
     g.V = D.V; g.NA = D.NA; g.NB = D.NB;
-    g.M = shfl_i(D.M, src);
-    g.T = shfl_u64(D.T, src);
-    g.HA = shfl_i(D.HA, src); g.HB = shfl_i(D.HB, src);
+    if constexpr (GATHER) {
+        const int src = (__lane_id() + sel) & 63;
+        g.M = shfl_i(D.M, src);
+        g.T = shfl_u64(D.T, src);
+        g.HA = shfl_i(D.HA, src); g.HB = shfl_i(D.HB, src);
+    } else {
+        g.M = pick3(D.M, sel);
+        g.T = ((uint64_t)(uint32_t)pick3((int)(uint32_t)(D.T >> 32), sel) << 32) |
+              (uint32_t)pick3((int)(uint32_t)D.T, sel);
+        g.HA = pick3(D.HA, sel); g.HB = pick3(D.HB, sel);
+    }
     return g;
 }
 
@@ -475,7 +491,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 const int j = (DIR > 0) ? (NCH - 1 - jj) : jj;
                 if (!live(j)) continue;
                 const int c = cN[j];
-                const int incl = (DIR > 0) ? wave_suffix_max(c) : wave_prefix_min(c);
+                const int incl = (DIR > 0) ? wave_suffix_max<NCH == 1>(c) : wave_prefix_min(c);
                 const int excl = (DIR > 0) ? lane_above_or(incl, SENT) : lane_below_or(incl, SENT);
                 const int prior = (DIR > 0) ? max(excl, carry) : min(excl, carry);
                 const bool rec = (DIR > 0) ? (c > prior) : (c < prior);
@@ -735,7 +751,7 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
                 else if (ac > ap || ac > am) sel = -1;
                 else sel = 0;
             }
-            Diag g = diag_pick(bd[j], sel);
+            Diag g = diag_pick<NCH == 1>(bd[j], sel);
             if (j > 0) {
                 const Diag e = diag_rdl(bd[j - 1], 63);
                 if (lane == 0 && sel == -1) g = e;

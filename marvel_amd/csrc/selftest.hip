@@ -66,19 +66,22 @@ __global__ __launch_bounds__(64) void st_match(const uint64_t* A, const uint64_t
     out[2 * i + 1] = mism ? 1 : 0;
 }
 
-// test 3: DPP lane moves; per lane: [lane_below(v), lane_above(v)]
+// test 3: DPP lane moves; per lane: [lane_below(v), lane_above(v), the
+// scans, the edge moves, the DPP suffix max, pick3 with sel = L % 3 - 1]
 __global__ __launch_bounds__(64) void st_dpp(const int* in, int* out)
 {
     const int lane = __lane_id();
     const int v = in[blockIdx.x * 64 + lane];
-    int* o = out + (blockIdx.x * 64 + lane) * 6;
+    int* o = out + (blockIdx.x * 64 + lane) * 8;
     o[0] = lane_below(v);
     o[1] = lane_above(v);
     // the aligner's scans: suffix max over values >= -1, prefix min over values >= 0
-    o[2] = wave_suffix_max(v);
+    o[2] = wave_suffix_max<true>(v);
     o[3] = wave_prefix_min(v < 0 ? 2147483647 : v);
     o[4] = lane_above_or(v, -7);
     o[5] = lane_below_or(v, -9);
+    o[6] = wave_suffix_max<false>(v);
+    o[7] = pick3(v, lane % 3 - 1);
 }
 
 }  // namespace mi

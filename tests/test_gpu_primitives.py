@@ -83,7 +83,9 @@ def test_dpp_wave_rotations_and_scans():
     """lane_below / lane_above (DPP wave_ror:1 / wave_rol:1) read lanes L-1 /
     L+1 mod 64; wave_suffix_max (values >= -1, the forward pass's
     antidiagonals and sentinel) and wave_prefix_min (values >= 0, the reverse
-    pass's, sentinel INT32_MAX) are the inclusive suffix max / prefix min."""
+    pass's, sentinel INT32_MAX) are the inclusive suffix max / prefix min,
+    both for the ds_bpermute suffix max (64-lane windows) and the DPP one
+    (wider windows); pick3 reads lane (L + sel) mod 64."""
     print("[prim] dpp", flush=True)
     rng = np.random.default_rng(9)
     waves = 8
@@ -94,7 +96,7 @@ def test_dpp_wave_rotations_and_scans():
     v[256:320] = -1                                      # all sentinels
     v[320::5] = -1                                       # scattered sentinels
     v[330] = 2**30
-    out = _run(3, inp=v, nout=waves * 64 * 6).reshape(waves, 64, 6)
+    out = _run(3, inp=v, nout=waves * 64 * 8).reshape(waves, 64, 8)
     for w in range(waves):
         x = v[w * 64:(w + 1) * 64]
         xr = np.where(x < 0, np.iinfo(np.int32).max, x).astype(np.int32)
@@ -104,3 +106,6 @@ def test_dpp_wave_rotations_and_scans():
         assert np.array_equal(out[w, :, 3], np.minimum.accumulate(xr))
         assert np.array_equal(out[w, :, 4], np.append(x[1:], -7))   # wave_shl:1, lane 63 keeps the edge value
         assert np.array_equal(out[w, :, 5], np.insert(x[:-1], 0, -9))   # wave_shr:1, lane 0 keeps it
+        assert np.array_equal(out[w, :, 6], np.maximum.accumulate(x[::-1])[::-1])
+        lanes = np.arange(64)
+        assert np.array_equal(out[w, :, 7], x[(lanes + lanes % 3 - 1) % 64])

@@ -242,12 +242,11 @@ int mi_las_merge(const void *body, long long size, long long novl, int tspace, v
 /* DBdust on the GPU: the SDUST scan of db/DBdust.c:236-470 over every read
  * of `db` (bases 0..3 at db->bases + reads[i].boff, as Read_All_Sequences
  * leaves them).  Unbiased scores: reads cut into segments of
- * MI_DUST_SEGMENT steps (default 384), one lane each after a warm-up of
- * MI_DUST_WARMUP steps (default max(64, 2 window)), segments that began in
- * another state than their predecessor ended in re-run; MI_DUST_STATS
- * prints the counts to stderr.  Biased scores and windows below 3: one
- * lane per read.
- * window (-w, 1..256), threshold (-t),
+ * MI_DUST_SEGMENT steps (default max(384, 4 window)), one lane each after
+ * a warm-up of MI_DUST_WARMUP steps (default max(64, 2 window)); segments
+ * that began in another state than their predecessor ended in are re-run;
+ * MI_DUST_STATS prints the counts to stderr.  Biased scores and windows
+ * below 3: one lane per read.  window (-w, 1..256), threshold (-t),
  * mlen (-m less one: intervals of at least mlen are kept), biased (-b, with
  * db->freq).  Writes anno[0..nreads] (byte offsets, as in the .dust.anno
  * file) and the points (pairs begin, end+1; at most the block's total read

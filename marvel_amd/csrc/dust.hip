@@ -614,7 +614,7 @@ int dust_segmented(const HITS_DB* db, const void* dbases, const std::vector<long
                    const std::vector<int>& rlen, int window, double threshold, int mlen, long long* anno, int* points)
 {
     const int n = db->nreads;
-    const int seglen = std::max(64, env_int("MI_DUST_SEGMENT", 384));
+    const int seglen = std::max(64, env_int("MI_DUST_SEGMENT", std::max(384, 4 * window)));   // warm-up <= half
     const int warm = std::max(0, env_int("MI_DUST_WARMUP", std::max(64, 2 * window)));
     std::vector<int> seg_read, seg_s0;
     for (int i = 0; i < n; i++)

@@ -20,6 +20,7 @@ CASES = {
     "rep_default": ("small_rep", 400, [], "C1", 0),
     "rep_w32_t15_m5": ("small_rep", 400, ["-w32", "-t1.5", "-m5"], "C1", 0),
     "rep_w100_t3": ("small_rep", 400, ["-w100", "-t3", "-m20"], "C1.db", 0),
+    "rep_w256_t12_m4": ("small_rep", 400, ["-w256", "-t1.2", "-m4"], "C1", 0),
     "rep_biased": ("small_rep", 400, ["-b"], "C1", 0),
     "rep_biased_w40": ("small_rep", 400, ["-b", "-w40", "-t1.2"], "C1", 0),
     "rep_block2": ("small_rep", 1, [], "C1.2", 0),

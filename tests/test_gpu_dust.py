@@ -27,7 +27,8 @@ def test_gpu_dbdust_matches_reference(case, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["rep_default", "rep_w100_t3", "rep_grow", "edges_default", "c2_default"])
+@pytest.mark.parametrize("case", ["rep_default", "rep_w100_t3", "rep_w256_t12_m4", "rep_grow", "edges_default",
+                                  "c2_default"])
 def test_gpu_dbdust_segment_rerun(case, tmp_path):
     assert os.path.exists(EXE), "DBdust not built (python -m marvel_amd.build)"
     print(f"[gpu DBdust] case {case}, 64-step segments, no warm-up", flush=True)

@@ -38,6 +38,10 @@ constexpr int kTrimLen = 15;
 constexpr int kTrimMask = 0x7fff;
 constexpr int kTrimMLag = 200;
 constexpr int kWaveLag = 30;
+// windows of several 64-lane chunks: a band of at most kOneChunk diagonals
+// that spans two chunks is moved into the middle of the first, so the
+// other chunks' steps are skipped (at least 12 diagonals of slack each side)
+constexpr int kOneChunk = 40;
 
 // status codes of one alignment
 constexpr int LA_OK = 0;
@@ -662,12 +666,16 @@ __device__ PassOut wave_pass(const DevSpec& sp, const uint64_t* A, int alen, con
         if (!elo) low += 1;
         const bool ehi = (hgh <= maxp);
         if (!ehi) hgh -= 1;
-        if (low < base || hgh > base + W - 1) {
-            if (hgh - low + 1 > W) {
-                out.status = LA_WIDE;
-                return out;
-            }
-            const int nbase = low - (W - (hgh - low + 1)) / 2;
+        const int wd = hgh - low + 1;
+        bool rebase = (low < base || hgh > base + W - 1);
+        if (rebase && wd > W) {
+            out.status = LA_WIDE;
+            return out;
+        }
+        if constexpr (NCH > 1)
+            if (wd <= kOneChunk && ((low - base) >> 6) != ((hgh - base) >> 6)) rebase = true;
+        if (rebase) {
+            const int nbase = (NCH > 1 && wd <= kOneChunk) ? low - (64 - wd) / 2 : low - (W - wd) / 2;
             const int delta = nbase - base;
             Diag nb[NCH];
 #pragma unroll

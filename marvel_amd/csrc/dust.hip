@@ -291,8 +291,9 @@ __global__ __launch_bounds__(64) void k_sdust(DustArgs a)
 // So a read is cut into segments of `seglen` steps, one lane each.  A
 // segment's lane starts `warm` steps early from the smallest state (lb = wb,
 // no candidate; lb only ever moves forward and the map from the previous lb
-// to the next is monotone, so the two runs meet once the true run's cuts
-// are inside the warm-up), records the state it reached at its first step,
+// to the next is monotone, so the warm-up's lb never passes the true one and
+// usually meets it within the warm-up: on the 5 Mbp reads every boundary
+// after 128 steps), records the state it reached at its first step,
 // then runs its own steps, folding the intervals it retires as the interval
 // stack does and appending them, and records its final state.  A second
 // kernel compares each segment's first state with its predecessor's final

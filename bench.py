@@ -53,6 +53,8 @@ GOLD_SHA = "2b379780f4257dd29f63fb4318baaa11b9d53ea3bf98de09e580c0401e0e363f"   
 HBM_PEAK_GBS = 8000.0
 # int32 VALU lane-operations: 256 CUs x 64 lanes x 2.4 GHz
 VALU_PEAK_GOPS = 256 * 64 * 2.4
+# wave64 VALU instruction issue: one per CU per cycle (each SIMD one every 4 cycles), G instructions/s
+VALU_ISSUE_PEAK_G = 256 * 2.4
 GATED_FLAGS = ["-k14", "-w6", "-h35", "-e.70", "-l1000", "-s100", "-j8", "-M32"]
 GATED_SHA = "8fb6b42443d1ed9a54b11316ad6e0d328cfce06719f0b89aad1a114ad94a9bd1"   # reference daligner, R.1 x R.1
 
@@ -93,6 +95,22 @@ def profiled_traffic(kernel):
         return None
     kib = lambda c: float(c.replace(",", ""))
     return int((kib(row[1]) * fetch_scale + kib(row[2]) * write_scale) * 1024)
+
+
+def profiled_valu_per_wave_step():
+    """VALU instructions per aligner wave step of k_report<1, false> from the committed counter summary
+    (SQ_INSTS_VALU per launch over the launch's wave steps).  None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_report_pmc_5mbp.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and cells[0] == "SQ_INSTS_VALU":
+            try:
+                return float(cells[2].split()[0])
+            except ValueError:
+                return None
+    return None
 
 
 def run_steps(dal, A, spec, dev, cA, d, name, steps, warmup):
@@ -432,6 +450,18 @@ def main():
         launches = sum(a["report_launches"] + b["report_launches"] for a, b in stats) / len(stats)
         w_align = stats[-1][0]["cells"] + stats[-1][0]["snake"] + stats[-1][1]["cells"] + stats[-1][1]["snake"]
         ach = w_align / k_report / 1e9 if k_report > 0 else 0.0
+        # the ceiling that binds it: VALU instruction issue (instructions per wave step from the counter summary,
+        # wave steps and kernel time of this run; the time includes the few wider-window launches)
+        vps = profiled_valu_per_wave_step()
+        wsteps = sum(x["wave_steps"] for x in stats[-1])
+        issue = None
+        if vps and k_report > 0:
+            ia = vps * wsteps / k_report / 1e9
+            issue = {"bound": "VALU instruction issue (one wave64 instruction per CU per cycle)",
+                     "achieved": round(ia, 1), "peak": VALU_ISSUE_PEAK_G, "unit": "G wave64 VALU instructions/s",
+                     "frac": round(ia / VALU_ISSUE_PEAK_G, 3), "valu_per_wave_step": vps,
+                     "wave_steps_per_step": wsteps,
+                     "source": "SQ_INSTS_VALU per wave step from profiles/k_report_pmc_5mbp.md"}
         # seed radix sort: 32 bytes per seed per 8-bit pass (read + write of a 16-byte seed)
         t_sort = sum(a["t_seed_sort"] + b["t_seed_sort"] for a, b in stats) / len(stats)
         sort_bytes = sum(x["nhits"] * 32 * x["seed_sort_passes"] for x in stats[-1])
@@ -492,7 +522,7 @@ def main():
                          "frac": round(ach / VALU_PEAK_GOPS, 5), "traffic": profiled_traffic("k_report<1, false>"),
                          "traffic_unit": "HBM bytes per k_report<1, false> launch (rocprofv3 --pmc, calibrated)",
                          "kernel_ms_per_step": round(k_report * 1e3, 2), "launches_per_step": launches,
-                         "w_align_per_step": w_align},
+                         "w_align_per_step": w_align, "issue": issue},
             "roofline_hbm": {"bound": "hbm", "kernel": "seed LSD radix sort (rs_hist + rs_scatter)",
                              "achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(sort_gbs / HBM_PEAK_GBS, 4), "traffic": sort_traffic,

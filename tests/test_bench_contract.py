@@ -32,3 +32,12 @@ def test_profiled_traffic_reads_the_committed_counters():
     # a seed-sort pass reads and writes at least the 71 M x 16-byte seeds once
     assert hist is not None and scat is not None and scat > 2 * 71e6 * 16 and hist > 71e6 * 16
     assert b.profiled_traffic("no_such_kernel") is None
+
+
+def test_profiled_valu_per_wave_step_reads_the_instruction_counters():
+    b = _bench()
+    v = b.profiled_valu_per_wave_step()
+    # SQ_INSTS_VALU per launch over the launch's 2.7e8 aligner wave steps: the committed summary's column
+    assert v is not None and 50 < v < 1000
+    # the issue ceiling the bench line compares against: one wave64 instruction per CU per cycle
+    assert abs(b.VALU_ISSUE_PEAK_G - 256 * 2.4) < 1e-9

@@ -493,6 +493,12 @@ __global__ __launch_bounds__(64) void k_sdust_seg(DustSegArgs a)
         double mscore = 0.;
         int dn = T(lb);
         for (c = lb; c > wb; c--) {
+            // the run's sum only grows toward the window's (at c = wb + 1 the
+            // run is the window), and thresh * (j - c) only grows as c falls:
+            // once that exceeds wsqr no position from here on can pass the
+            // test below, and the positions not added need no undo (most
+            // walks end here at c = lb)
+            if (a.thresh * (j - c) > wsqr) break;
             const int d = dn;
             dn = T(c - 1);
             lsqr += LC(d)++;

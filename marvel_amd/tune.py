@@ -25,7 +25,7 @@ FIELDS = ("report_waves_per_cu", "report_pool_log2", "report_l1_waves_per_cu", "
 DEFAULT = {"report_waves_per_cu": 32, "report_pool_log2": 15, "report_l1_waves_per_cu": 16, "batch_waves_per_cu": 32,
            "batch_wide_waves_per_cu": 8}
 REPORT_CANDIDATES = [(w, p) for w in (16, 24, 32) for p in (14, 15, 16)]
-L1_CANDIDATES = (8, 16, 32)
+L1_CANDIDATES = (8, 16, 24, 32)
 # (64-diagonal window, wider windows): the levels are timed apart, so each
 # run scores one candidate of each and the two are chosen independently
 BATCH_CANDIDATES = ((16, 8), (24, 12), (32, 16), (32, 20))
@@ -92,7 +92,13 @@ def _time_report(step, reps=1) -> float:
 def tune_report(step) -> dict:
     """Picks the report kernel's shapes with `step()` (one block-pair
     comparison returning both passes' stats): waves per CU and pool at the
-    64-diagonal window, then waves per CU at the 128-diagonal window."""
+    64-diagonal window, then waves per CU at the 128-diagonal window.  The
+    latter are timed with every pair started at the 128-diagonal window
+    (MI_REPORT_FIRST_LEVEL=1): in an ordinary step that window re-runs only
+    the few pairs that outgrow 64 diagonals, which no candidate changes
+    measurably, while on repeat-rich blocks it carries as much work as the
+    64-diagonal window and the shape matters (8 waves per CU ran 19 % slower
+    than 16 to 32 there)."""
     base = current()
     timings = {}
     for w, p in REPORT_CANDIDATES:
@@ -100,9 +106,17 @@ def tune_report(step) -> dict:
         timings[f"w{w}_p{p}"] = _time_report(step)
     w, p = min(REPORT_CANDIDATES, key=lambda c: timings[f"w{c[0]}_p{c[1]}"])
     best = {**base, "report_waves_per_cu": w, "report_pool_log2": p}
-    for l1 in L1_CANDIDATES:
-        apply({**best, "report_l1_waves_per_cu": l1})
-        timings[f"l1_{l1}"] = _time_report(step)
+    prev = os.environ.get("MI_REPORT_FIRST_LEVEL")
+    os.environ["MI_REPORT_FIRST_LEVEL"] = "1"
+    try:
+        for l1 in L1_CANDIDATES:
+            apply({**best, "report_l1_waves_per_cu": l1})
+            timings[f"l1_{l1}"] = _time_report(step)
+    finally:
+        if prev is None:
+            del os.environ["MI_REPORT_FIRST_LEVEL"]
+        else:
+            os.environ["MI_REPORT_FIRST_LEVEL"] = prev
     best["report_l1_waves_per_cu"] = min(L1_CANDIDATES, key=lambda c: timings[f"l1_{c}"])
     apply(best)
     return {"choice": best, "timings_s": {k: round(v, 5) for k, v in timings.items()}}

@@ -272,7 +272,7 @@ int mi_device_memset(void *p, int value, long long bytes);
  * 16), the pebble pool of the 64-diagonal window as a power of two (12..18,
  * default 15), waves per CU of the batched aligner's 64-diagonal window
  * (4..32, default 32) and of its 128- and 256-diagonal windows (2..32,
- * default 8).  A 0 field keeps the default; NULL restores all defaults.
+ * default 16).  A 0 field keeps the default; NULL restores all defaults.
  * The tuner in marvel_amd/tune.py picks them from a calibration run. */
 typedef struct {
     int report_waves_per_cu;

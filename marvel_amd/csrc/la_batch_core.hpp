@@ -248,8 +248,8 @@ static BatchPlan make_plan(int maxlen, int tspace, int cus)
     // windows of 64, 128, 256 and 1,024 diagonals; a task re-runs at the
     // next level when its band outgrows the window or its pool fills
     p.lv[0] = Level{1, 1 << 15, cus * (w0 ? w0 : 32)};   // default 8 waves per SIMD
-    p.lv[1] = Level{2, 1 << 17, cus * (w1 ? w1 : 8)};
-    p.lv[2] = Level{4, 1 << 17, cus * (w1 ? w1 : 8)};
+    p.lv[1] = Level{2, 1 << 17, cus * (w1 ? w1 : 16)};
+    p.lv[2] = Level{4, 1 << 17, cus * (w1 ? w1 : 16)};
     p.lv[3] = Level{16, 1 << 20, 256};
     return p;
 }

@@ -23,7 +23,7 @@ from .lib import MiTuning, lib
 FIELDS = ("report_waves_per_cu", "report_pool_log2", "report_l1_waves_per_cu", "batch_waves_per_cu",
           "batch_wide_waves_per_cu")
 DEFAULT = {"report_waves_per_cu": 32, "report_pool_log2": 15, "report_l1_waves_per_cu": 16, "batch_waves_per_cu": 32,
-           "batch_wide_waves_per_cu": 8}
+           "batch_wide_waves_per_cu": 16}
 REPORT_CANDIDATES = [(w, p) for w in (16, 24, 32) for p in (14, 15, 16)]
 L1_CANDIDATES = (8, 16, 24, 32)
 # (64-diagonal window, wider windows): the levels are timed apart, so each

@@ -780,6 +780,17 @@ struct EvPair {
 };
 static EvPair g_ev_report, g_ev_sort;
 
+// The report kernel's first window for the next pass.  Every window computes
+// the same alignments, so this only decides speed: a pass that starts at 64
+// diagonals re-runs the pairs that outgrow them at 128 after a partial run,
+// which pays on blocks where few pairs do; on repeat-rich blocks, where the
+// 128-diagonal re-runs take longer than the whole 64-diagonal launch, the
+// next passes start at 128 (kReprobe of them, then one starts at 64 again to
+// re-measure).  Match_Filter runs one call at a time, as the reference's.
+static int g_report_first = 0, g_report_reprobe = 0;
+constexpr double kWideOverNarrow = 1.0;
+constexpr int kReprobe = 8;
+
 struct Level {
     int nch, pool_cap, mcap;
     long long mtcap;
@@ -1244,6 +1255,7 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
     }
 
     int cur_level = -1, gen = 0;
+    double t_level[kLevels] = {0., 0., 0., 0., 0.};   // report kernel seconds per window of this pass
 
     // runs the n jobs listed at dorder (device) at `level`; returns the
     // (job, status) pairs that need another launch
@@ -1305,8 +1317,10 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         if (wd > 0) throw Fail("report kernel exceeded the watchdog deadline and was stopped", MI_ERR_TIMEOUT);
         if (wd < 0) throw Fail("report kernel failed on the device");
         HC(hipGetLastError());
-        S.t_report_kernel += g_ev_report.seconds();
+        const double sec = g_ev_report.seconds();
+        S.t_report_kernel += sec;
         S.report_launches += 1;
+        t_level[level] += sec;
         u64 oc2[2];
         int nf = 0;
         HC(hipMemcpy(oc2, ocount.p, sizeof oc2, hipMemcpyDeviceToHost));
@@ -1347,9 +1361,11 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
 
     // runs a job list until every job is done: output-arena overflows are
     // re-run at the same window, wide bands / full pools at the next one.
-    // MI_REPORT_FIRST_LEVEL starts every job at a wider window instead.
-    int first_level = 0;
-    if (const char* e = getenv("MI_REPORT_FIRST_LEVEL")) first_level = std::min(std::max(atoi(e), 0), kLevels - 1);
+    // Jobs start at g_report_first; MI_REPORT_FIRST_LEVEL starts every job at
+    // the window it names instead (and keeps the choice fixed).
+    int first_level = g_report_first;
+    const char* fixed_first = getenv("MI_REPORT_FIRST_LEVEL");
+    if (fixed_first) first_level = std::min(std::max(atoi(fixed_first), 0), kLevels - 1);
     auto run_all = [&](const int* dorder, long long n) {
         int level = first_level;
         std::vector<std::pair<int, int>> again = run(dorder, n, level);
@@ -1371,6 +1387,16 @@ static void match_pass(KIdx* A, KIdx* B, bool self, int comp, bool same_index, S
         }
     };
     run_all(order_all.p, nfirst);
+    if (!fixed_first) {
+        if (first_level == 0) {
+            if (t_level[0] > 0. && t_level[1] > kWideOverNarrow * t_level[0]) {
+                g_report_first = 1;
+                g_report_reprobe = kReprobe;
+            }
+        } else if (--g_report_reprobe <= 0) {
+            g_report_first = 0;
+        }
+    }
 
     // the cell shared by lastp[maxdiag] and lasta[mindiag-1] carries from
     // one pair to the next within a report chunk: re-run the pairs that

@@ -89,6 +89,26 @@ def _time_report(step, reps=1) -> float:
     return best
 
 
+class _FirstLevel:
+    """Pins the report kernel's first window (MI_REPORT_FIRST_LEVEL) for a
+    calibration loop, so the library's own choice of it does not move
+    between candidates."""
+
+    def __init__(self, level: int):
+        self.level = str(level)
+
+    def __enter__(self):
+        self.prev = os.environ.get("MI_REPORT_FIRST_LEVEL")
+        os.environ["MI_REPORT_FIRST_LEVEL"] = self.level
+
+    def __exit__(self, *exc):
+        if self.prev is None:
+            del os.environ["MI_REPORT_FIRST_LEVEL"]
+        else:
+            os.environ["MI_REPORT_FIRST_LEVEL"] = self.prev
+        return False
+
+
 def tune_report(step) -> dict:
     """Picks the report kernel's shapes with `step()` (one block-pair
     comparison returning both passes' stats): waves per CU and pool at the
@@ -101,22 +121,16 @@ def tune_report(step) -> dict:
     than 16 to 32 there)."""
     base = current()
     timings = {}
-    for w, p in REPORT_CANDIDATES:
-        apply({**base, "report_waves_per_cu": w, "report_pool_log2": p})
-        timings[f"w{w}_p{p}"] = _time_report(step)
+    with _FirstLevel(0):
+        for w, p in REPORT_CANDIDATES:
+            apply({**base, "report_waves_per_cu": w, "report_pool_log2": p})
+            timings[f"w{w}_p{p}"] = _time_report(step)
     w, p = min(REPORT_CANDIDATES, key=lambda c: timings[f"w{c[0]}_p{c[1]}"])
     best = {**base, "report_waves_per_cu": w, "report_pool_log2": p}
-    prev = os.environ.get("MI_REPORT_FIRST_LEVEL")
-    os.environ["MI_REPORT_FIRST_LEVEL"] = "1"
-    try:
+    with _FirstLevel(1):
         for l1 in L1_CANDIDATES:
             apply({**best, "report_l1_waves_per_cu": l1})
             timings[f"l1_{l1}"] = _time_report(step)
-    finally:
-        if prev is None:
-            del os.environ["MI_REPORT_FIRST_LEVEL"]
-        else:
-            os.environ["MI_REPORT_FIRST_LEVEL"] = prev
     best["report_l1_waves_per_cu"] = min(L1_CANDIDATES, key=lambda c: timings[f"l1_{c}"])
     apply(best)
     return {"choice": best, "timings_s": {k: round(v, 5) for k, v in timings.items()}}
